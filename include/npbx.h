@@ -1,0 +1,129 @@
+/*
+ * npbx.h -- C ABI of libnpbx.so, the MI355X (gfx950) structured-grid stencil
+ * library behind the npbench "mi355x" framework entry.
+ *
+ * Conventions
+ *  - Every entry point returns int: NPBX_OK (0) or a negative npbx_status.
+ *    npbx_last_error() copies the calling thread's last message.
+ *  - Array arguments are device pointers (HBM) to C-contiguous row-major
+ *    arrays, last axis fastest, exactly the NumPy layouts of the reference
+ *    kernels.  Sizes are element counts.  `stream` is a hipStream_t (NULL =
+ *    the default stream); kernels are enqueued and the call returns without
+ *    waiting; use npbx_sync() to wait with a deadline.
+ *  - dtype selects the storage type: NPBX_F64 (the drop-in, FP64 arithmetic,
+ *    bitwise equal to the reference NumPy code), NPBX_F32 and NPBX_BF16
+ *    (reduced-precision storage, FP32 arithmetic).  vadv always eliminates
+ *    in FP64, whatever the field storage type.
+ *
+ * Reference interfaces replaced (paths relative to the npbench source tree):
+ *  - npbx_jacobi2d  <- npbench/benchmarks/polybench/jacobi_2d/jacobi_2d_numpy.py:4
+ *                      kernel(TSTEPS, A, B)
+ *  - npbx_heat3d    <- npbench/benchmarks/polybench/heat_3d/heat_3d_numpy.py:4
+ *                      kernel(TSTEPS, A, B)
+ *  - npbx_hdiff     <- npbench/benchmarks/weather_stencils/hdiff/hdiff_numpy.py:5
+ *                      hdiff(in_field, out_field, coeff)
+ *  - npbx_vadv      <- npbench/benchmarks/weather_stencils/vadv/vadv_numpy.py:9
+ *                      vadv(utens_stage, u_stage, wcon, u_pos, utens, dtr_stage)
+ *  - copy / sync / timing entry points back the framework adapter's
+ *    copy_func / copy_back_func / exec_str synchronisation
+ *    (npbench/infrastructure/framework.py:42-50,140-162,
+ *     npbench/infrastructure/cupy_framework.py:26-58).
+ *  - The batch entry points have no reference counterpart: they run many
+ *    independent grids of uneven size in one launch for the batched and
+ *    serving forms of the same kernels.
+ */
+#ifndef NPBX_H
+#define NPBX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum npbx_status {
+  NPBX_OK = 0,
+  NPBX_EINVAL = -1,       /* bad argument (shape, pointer, dtype, aliasing) */
+  NPBX_EHIP = -2,         /* a HIP runtime call failed */
+  NPBX_ETIMEDOUT = -3,    /* npbx_sync deadline passed; work still queued */
+  NPBX_ENOMEM = -4,       /* device allocation failed */
+  NPBX_EUNSUPPORTED = -5  /* valid request this build does not implement */
+} npbx_status;
+
+typedef enum npbx_dtype {
+  NPBX_F64 = 0,
+  NPBX_F32 = 1,
+  NPBX_BF16 = 2
+} npbx_dtype;
+
+/* ---- library / device ---------------------------------------------------- */
+int npbx_version(char *buf, size_t len);
+int npbx_last_error(char *buf, size_t len);
+int npbx_device_count(int *count);
+/* Selects the device for this thread and checks it is a gfx950 part. */
+int npbx_init(int device);
+
+/* ---- memory, streams, events (framework.py:42-50 copy_func/copy_back_func) */
+int npbx_malloc(void **ptr, size_t bytes);
+int npbx_free(void *ptr);
+int npbx_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int npbx_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int npbx_stream_create(void **stream);
+int npbx_stream_destroy(void *stream);
+/* Waits for `stream` until it drains or the monotonic clock passes
+ * deadline_ns (CLOCK_MONOTONIC nanoseconds; 0 = no deadline).  Polls, so a
+ * host-side timeout can interrupt between calls (timeout_decorator.py:46-72). */
+int npbx_sync(void *stream, int64_t deadline_ns);
+int npbx_event_create(void **event);
+int npbx_event_destroy(void *event);
+int npbx_event_record(void *event, void *stream);
+int npbx_event_elapsed_ms(void *start, void *stop, float *ms);
+
+/* ---- storage conversion ---------------------------------------------------
+ * Element-wise dst = convert(src), round-to-nearest-even (FP64->BF16 is
+ * rounded once, not twice).  No reference counterpart: used by the FP32 and
+ * BF16 storage forms of the drop-in kernels. */
+int npbx_cast(int src_dtype, const void *src, int dst_dtype, void *dst,
+              int64_t count, void *stream);
+
+/* ---- 2-D 5-point Jacobi (jacobi_2d_numpy.py:4-10) -------------------------
+ * A, B: N x N.  Runs TSTEPS-1 ping-pong iterations (2*(TSTEPS-1) sweeps,
+ * A->B then B->A), interior only, boundaries never written.
+ * tblock: sweeps fused per HBM round trip (0 or 1 = one launch per sweep).
+ */
+int npbx_jacobi2d(int dtype, void *A, void *B, int64_t N, int64_t tsteps,
+                  int tblock, void *stream);
+/* Same sweep, counted in sweeps: sweep s (s = first..first+nsweeps-1) reads
+ * A and writes B when s is even, and reads B and writes A when s is odd. */
+int npbx_jacobi2d_sweeps(int dtype, void *A, void *B, int64_t N,
+                         int64_t first, int64_t nsweeps, int tblock,
+                         void *stream);
+
+/* ---- 3-D 7-point heat (heat_3d_numpy.py:4-20) ----------------------------- */
+int npbx_heat3d(int dtype, void *A, void *B, int64_t N, int64_t tsteps,
+                int tblock, void *stream);
+int npbx_heat3d_sweeps(int dtype, void *A, void *B, int64_t N, int64_t first,
+                       int64_t nsweeps, int tblock, void *stream);
+
+/* ---- horizontal diffusion (hdiff_numpy.py:5-29) ----------------------------
+ * in_field: (I+4, J+4, K); out_field, coeff: (I, J, K).  nsteps == 1 is the
+ * reference; nsteps > 1 iterates, feeding out back into the interior of the
+ * input with its 2-cell halo held fixed (in_field itself is not modified). */
+int npbx_hdiff(int dtype, const void *in_field, void *out_field,
+               const void *coeff, int64_t I, int64_t J, int64_t K,
+               int64_t nsteps, void *stream);
+
+/* ---- implicit vertical advection (vadv_numpy.py:9-78) ----------------------
+ * utens_stage (in/out), u_stage, u_pos, utens: (I, J, K); wcon: (I+1, J, K).
+ * field_dtype NPBX_F64 or NPBX_F32; the elimination is FP64 in both.  K >= 2.
+ */
+int npbx_vadv(int field_dtype, void *utens_stage, const void *u_stage,
+              const void *wcon, const void *u_pos, const void *utens,
+              double dtr_stage, int64_t I, int64_t J, int64_t K, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NPBX_H */

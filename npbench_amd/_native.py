@@ -1,0 +1,115 @@
+"""ctypes binding of libnpbx.so (the C ABI declared in include/npbx.h).
+
+The library is required: importing the kernels without it raises
+``NativeLibraryMissing`` -- there is no CPU fallback.
+"""
+
+import ctypes
+import pathlib
+import threading
+
+LIB_PATH = pathlib.Path(__file__).resolve().parent / "libnpbx.so"
+
+OK = 0
+EINVAL = -1
+EHIP = -2
+ETIMEDOUT = -3
+ENOMEM = -4
+EUNSUPPORTED = -5
+
+F64 = 0
+F32 = 1
+BF16 = 2
+
+_STATUS = {EINVAL: "EINVAL", EHIP: "EHIP", ETIMEDOUT: "ETIMEDOUT", ENOMEM: "ENOMEM",
+           EUNSUPPORTED: "EUNSUPPORTED"}
+
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_sz = ctypes.c_size_t
+
+# name -> argtypes (every function returns int)
+SIGNATURES = {
+    "npbx_version": [ctypes.c_char_p, _sz],
+    "npbx_last_error": [ctypes.c_char_p, _sz],
+    "npbx_device_count": [ctypes.POINTER(_int)],
+    "npbx_init": [_int],
+    "npbx_malloc": [ctypes.POINTER(_vp), _sz],
+    "npbx_free": [_vp],
+    "npbx_memcpy_h2d": [_vp, _vp, _sz, _vp],
+    "npbx_memcpy_d2h": [_vp, _vp, _sz, _vp],
+    "npbx_stream_create": [ctypes.POINTER(_vp)],
+    "npbx_stream_destroy": [_vp],
+    "npbx_sync": [_vp, _i64],
+    "npbx_event_create": [ctypes.POINTER(_vp)],
+    "npbx_event_destroy": [_vp],
+    "npbx_event_record": [_vp, _vp],
+    "npbx_event_elapsed_ms": [_vp, _vp, ctypes.POINTER(ctypes.c_float)],
+    "npbx_cast": [_int, _vp, _int, _vp, _i64, _vp],
+    "npbx_jacobi2d": [_int, _vp, _vp, _i64, _i64, _int, _vp],
+    "npbx_jacobi2d_sweeps": [_int, _vp, _vp, _i64, _i64, _i64, _int, _vp],
+    "npbx_heat3d": [_int, _vp, _vp, _i64, _i64, _int, _vp],
+    "npbx_heat3d_sweeps": [_int, _vp, _vp, _i64, _i64, _i64, _int, _vp],
+    "npbx_hdiff": [_int, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _vp],
+    "npbx_vadv": [_int, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _i64, _i64, _i64, _vp],
+}
+
+
+class NativeLibraryMissing(RuntimeError):
+    """libnpbx.so has not been built (run ``python -m npbench_amd.build``)."""
+
+
+class NpbxError(RuntimeError):
+    def __init__(self, fn, code, message):
+        self.fn = fn
+        self.code = code
+        super().__init__(f"{fn} -> {_STATUS.get(code, code)}: {message}")
+
+
+class NpbxTimeout(NpbxError):
+    pass
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def lib():
+    """Loads libnpbx.so once and declares the prototypes."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise NativeLibraryMissing(
+                    f"{LIB_PATH} is missing: build it with `python -m npbench_amd.build`")
+            handle = ctypes.CDLL(str(LIB_PATH))
+            for name, argtypes in SIGNATURES.items():
+                fn = getattr(handle, name)
+                fn.argtypes = argtypes
+                fn.restype = _int
+            _lib = handle
+    return _lib
+
+
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(512)
+    lib().npbx_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def call(name, *args):
+    """Calls ``name`` and raises NpbxError (or NpbxTimeout) on failure."""
+    rc = getattr(lib(), name)(*args)
+    if rc != OK:
+        cls = NpbxTimeout if rc == ETIMEDOUT else NpbxError
+        raise cls(name, rc, last_error())
+    return rc
+
+
+def version() -> str:
+    buf = ctypes.create_string_buffer(64)
+    call("npbx_version", buf, len(buf))
+    return buf.value.decode()

@@ -1,0 +1,119 @@
+// Shared declarations of libnpbx: error reporting, storage types and launch
+// helpers.  All device code targets gfx950 (CDNA4, wave64) only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "npbx.h"
+
+namespace npbx {
+
+// ---- error reporting (thread-local last message, see npbx_last_error) ----
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int hip_fail(hipError_t err, const char* what, const char* file, int line);
+void clear_error();
+
+#define NPBX_HIP(call)                                                     \
+  do {                                                                     \
+    hipError_t npbx_e_ = (call);                                           \
+    if (npbx_e_ != hipSuccess)                                             \
+      return ::npbx::hip_fail(npbx_e_, #call, __FILE__, __LINE__);         \
+  } while (0)
+
+#define NPBX_REQUIRE(cond, ...)                                            \
+  do {                                                                     \
+    if (!(cond)) return ::npbx::fail(NPBX_EINVAL, __VA_ARGS__);            \
+  } while (0)
+
+#define NPBX_TRY(expr)                                                     \
+  do {                                                                     \
+    int npbx_rc_ = (expr);                                                 \
+    if (npbx_rc_ != NPBX_OK) return npbx_rc_;                              \
+  } while (0)
+
+// Checks the launch that was just enqueued.
+int check_launch(const char* kernel, const char* file, int line);
+#define NPBX_LAUNCHED(name) NPBX_TRY(::npbx::check_launch(name, __FILE__, __LINE__))
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// ---- storage types -------------------------------------------------------
+// bf16 is kept as its 16-bit pattern; arithmetic happens in float.
+struct bf16 {
+  uint16_t bits;
+};
+
+// Round-to-nearest-even float -> bf16 from the IEEE-754 definition: add
+// half an ulp of the target (minus one when the kept lsb is 0, so ties go to
+// even) and truncate.  NaNs stay NaN (quiet bit forced).
+__device__ __forceinline__ uint16_t f32_to_bf16_bits(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<uint16_t>((u >> 16) | 0x0040u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return static_cast<uint16_t>(u >> 16);
+}
+
+template <class S>
+struct Storage;
+
+template <>
+struct Storage<double> {
+  using C = double;  // arithmetic type
+  static __device__ __forceinline__ double load(double v) { return v; }
+  static __device__ __forceinline__ double store(double v) { return v; }
+};
+
+template <>
+struct Storage<float> {
+  using C = float;
+  static __device__ __forceinline__ float load(float v) { return v; }
+  static __device__ __forceinline__ float store(float v) { return v; }
+};
+
+template <>
+struct Storage<bf16> {
+  using C = float;
+  static __device__ __forceinline__ float load(bf16 v) {
+    return __uint_as_float(static_cast<uint32_t>(v.bits) << 16);
+  }
+  static __device__ __forceinline__ bf16 store(float v) { return bf16{f32_to_bf16_bits(v)}; }
+};
+
+inline size_t dtype_size(int dtype) {
+  switch (dtype) {
+    case NPBX_F64: return 8;
+    case NPBX_F32: return 4;
+    case NPBX_BF16: return 2;
+    default: return 0;
+  }
+}
+
+inline const char* dtype_name(int dtype) {
+  switch (dtype) {
+    case NPBX_F64: return "f64";
+    case NPBX_F32: return "f32";
+    case NPBX_BF16: return "bf16";
+    default: return "?";
+  }
+}
+
+// Calls f(S{}) with S the storage type for dtype; EINVAL for unknown dtypes.
+template <class F>
+int dispatch_dtype(int dtype, F&& f) {
+  switch (dtype) {
+    case NPBX_F64: return f(double{});
+    case NPBX_F32: return f(float{});
+    case NPBX_BF16: return f(bf16{});
+    default: return fail(NPBX_EINVAL, "unknown dtype %d", dtype);
+  }
+}
+
+// Per-device scratch owned by the library (grow-only, freed at exit).  Used
+// by kernels that need storage beyond their inputs and outputs.
+int workspace(size_t bytes, void** ptr);
+
+}  // namespace npbx

@@ -1,0 +1,175 @@
+"""Pure-Python scalar-loop restatement of the four kernels (tiny shapes only).
+
+Test infrastructure only (see ``oracle/__init__.py``).  This is a second,
+structurally different statement of the reference arithmetic: it works point
+by point on Python floats (IEEE binary64, round-to-nearest) with explicit
+index arithmetic instead of NumPy slices, so an off-by-one slice or a swapped
+operand in ``oracle.stencils`` shows up as a bit difference.  Each function
+returns new arrays and leaves its inputs alone.
+
+References: jacobi_2d_numpy.py:4-10, heat_3d_numpy.py:4-20,
+hdiff_numpy.py:5-29, vadv_numpy.py:4-78 (npbench/benchmarks/...).
+"""
+
+import numpy as np
+
+
+def jacobi_sweeps(A, B, nsweeps):
+    n = A.shape[0]
+    a = A.tolist()
+    b = B.tolist()
+    for s in range(nsweeps):
+        src, dst = (a, b) if s % 2 == 0 else (b, a)
+        new = [row[:] for row in dst]
+        for i in range(1, n - 1):
+            for j in range(1, n - 1):
+                t = src[i][j]
+                t = t + src[i][j - 1]
+                t = t + src[i][j + 1]
+                t = t + src[i + 1][j]
+                t = t + src[i - 1][j]
+                new[i][j] = 0.2 * t
+        if s % 2 == 0:
+            b = new
+        else:
+            a = new
+    return np.array(a), np.array(b)
+
+
+def jacobi_2d_kernel(TSTEPS, A, B):
+    return jacobi_sweeps(A, B, 2 * max(TSTEPS - 1, 0))
+
+
+def heat_sweeps(A, B, nsweeps):
+    n = A.shape[0]
+    a = A.tolist()
+    b = B.tolist()
+
+    def axis(up, centre, down):
+        return 0.125 * ((up - 2.0 * centre) + down)
+
+    for s in range(nsweeps):
+        src, dst = (a, b) if s % 2 == 0 else (b, a)
+        new = [[row[:] for row in plane] for plane in dst]
+        for i in range(1, n - 1):
+            for j in range(1, n - 1):
+                for k in range(1, n - 1):
+                    c = src[i][j][k]
+                    ti = axis(src[i + 1][j][k], c, src[i - 1][j][k])
+                    tj = axis(src[i][j + 1][k], c, src[i][j - 1][k])
+                    tk = axis(src[i][j][k + 1], c, src[i][j][k - 1])
+                    new[i][j][k] = ((ti + tj) + tk) + c
+        if s % 2 == 0:
+            b = new
+        else:
+            a = new
+    return np.array(a), np.array(b)
+
+
+def heat_3d_kernel(TSTEPS, A, B):
+    return heat_sweeps(A, B, 2 * max(TSTEPS - 1, 0))
+
+
+def hdiff(in_field, coeff):
+    """Return out (I, J, K).  Coordinates: output (x, y) sits at in[x+2, y+2]."""
+    I, J, K = coeff.shape
+    f = in_field.tolist()
+    cf = coeff.tolist()
+
+    def lap(p, q, k):
+        # Laplacian centred on in-field point (p, q)
+        nb = f[p + 1][q][k] + f[p - 1][q][k]
+        nb = nb + f[p][q + 1][k]
+        nb = nb + f[p][q - 1][k]
+        return 4.0 * f[p][q][k] - nb
+
+    def limit(r, grad):
+        return 0.0 if r * grad > 0 else r
+
+    def flux_i(p, q, k):
+        # flux across the face between in-field points (p, q) and (p+1, q)
+        r = lap(p + 1, q, k) - lap(p, q, k)
+        return limit(r, f[p + 1][q][k] - f[p][q][k])
+
+    def flux_j(p, q, k):
+        # flux across the face between in-field points (p, q) and (p, q+1)
+        r = lap(p, q + 1, k) - lap(p, q, k)
+        return limit(r, f[p][q + 1][k] - f[p][q][k])
+
+    out = np.empty((I, J, K))
+    for x in range(I):
+        for y in range(J):
+            for k in range(K):
+                p, q = x + 2, y + 2
+                d = flux_i(p, q, k) - flux_i(p - 1, q, k)
+                d = d + flux_j(p, q, k)
+                d = d - flux_j(p, q - 1, k)
+                out[x, y, k] = f[p][q][k] - cf[x][y][k] * d
+    return out
+
+
+def hdiff_iterated(in_field, coeff, nsteps):
+    I, J, _ = coeff.shape
+    work = np.array(in_field, dtype=np.float64, copy=True)
+    out = None
+    for _ in range(nsteps):
+        out = hdiff(work, coeff)
+        work[2:I + 2, 2:J + 2] = out
+    return out
+
+
+def _thomas(a, b, c, d):
+    """Thomas algorithm with the reference's reciprocal-then-multiply pivots."""
+    n = len(b)
+    cp = [0.0] * n
+    dp = [0.0] * n
+    inv = 1.0 / b[0]
+    cp[0] = c[0] * inv
+    dp[0] = d[0] * inv
+    for k in range(1, n):
+        inv = 1.0 / (b[k] - cp[k - 1] * a[k])
+        cp[k] = c[k] * inv
+        dp[k] = (d[k] - dp[k - 1] * a[k]) * inv
+    x = [0.0] * n
+    x[n - 1] = dp[n - 1]
+    for k in range(n - 2, -1, -1):
+        x[k] = dp[k] - cp[k] * x[k + 1]
+    return x
+
+
+def vadv(utens_stage, u_stage, wcon, u_pos, utens, dtr_stage):
+    """Return the new utens_stage (I, J, K)."""
+    I, J, K = utens_stage.shape
+    out = np.empty((I, J, K))
+    for i in range(I):
+        for j in range(J):
+            u = u_stage[i, j].tolist()
+            s = [wcon[i + 1, j, k] + wcon[i, j, k] for k in range(K)]
+            a = [0.0] * K
+            b = [0.0] * K
+            c = [0.0] * K
+            d = [0.0] * K
+            for k in range(K):
+                base = ((dtr_stage * u_pos[i, j, k] + utens[i, j, k]) +
+                        utens_stage[i, j, k])
+                has_lo = k > 0
+                has_hi = k < K - 1
+                gav = -0.25 * s[k] if has_lo else 0.0
+                gcv = 0.25 * s[k + 1] if has_hi else 0.0
+                a[k] = gav * 0.5
+                c[k] = gcv * 0.5
+                if has_lo and has_hi:
+                    b[k] = dtr_stage - a[k] - c[k]
+                    corr = (-(gav * 0.5) * (u[k - 1] - u[k]) -
+                            (gcv * 0.5) * (u[k + 1] - u[k]))
+                elif has_hi:
+                    b[k] = dtr_stage - c[k]
+                    corr = -(gcv * 0.5) * (u[k + 1] - u[k])
+                else:
+                    b[k] = dtr_stage - a[k]
+                    corr = -(gav * 0.5) * (u[k - 1] - u[k])
+                d[k] = base + corr
+            x = _thomas(a, b, c, d)
+            for k in range(K):
+                out[i, j, k] = dtr_stage * (x[k] - u_pos[i, j, k])
+    return out

@@ -1,0 +1,235 @@
+"""Vectorised NumPy FP64 restatement of the reference stencil kernels.
+
+Test infrastructure only (see ``oracle/__init__.py``).  Each function keeps
+the element-wise order of operations of the reference NumPy code it follows,
+so that on identical FP64 inputs the results are bit-identical to it.
+
+Reference files (paths relative to the npbench source tree):
+  jacobi_2d_kernel  npbench/benchmarks/polybench/jacobi_2d/jacobi_2d_numpy.py:4-10
+  heat_3d_kernel    npbench/benchmarks/polybench/heat_3d/heat_3d_numpy.py:4-20
+  hdiff             npbench/benchmarks/weather_stencils/hdiff/hdiff_numpy.py:5-29
+  vadv              npbench/benchmarks/weather_stencils/vadv/vadv_numpy.py:4-78
+"""
+
+import numpy as np
+
+
+def _f64(*arrays):
+    for a in arrays:
+        if not (isinstance(a, np.ndarray) and a.dtype == np.float64):
+            raise TypeError("the oracle computes in float64 and needs float64 "
+                            "ndarrays")
+
+
+# vadv_numpy.py:4-5 -- implicit and explicit weights of the vertical scheme.
+BET_M = 0.5
+BET_P = 0.5
+
+
+# --------------------------------------------------------------------------
+# 2-D 5-point Jacobi
+# --------------------------------------------------------------------------
+def _jacobi_sweep(src, dst):
+    """One sweep ``dst[interior] = 0.2 * (c + west + east + south + north)``.
+
+    Follows jacobi_2d_numpy.py:7-8: the sum is accumulated left to right as
+    ``((((c + W) + E) + S) + N)`` with W = [i, j-1], E = [i, j+1],
+    S = [i+1, j], N = [i-1, j], then scaled by 0.2.  Boundary rows and
+    columns of ``dst`` are never written.
+    """
+    c = src[1:-1, 1:-1]
+    acc = c + src[1:-1, :-2]
+    acc = acc + src[1:-1, 2:]
+    acc = acc + src[2:, 1:-1]
+    acc = acc + src[:-2, 1:-1]
+    dst[1:-1, 1:-1] = 0.2 * acc
+
+
+def jacobi_sweeps(A, B, nsweeps):
+    """Run ``nsweeps`` alternating sweeps: odd sweeps A->B, even sweeps B->A."""
+    _f64(A, B)
+    for s in range(nsweeps):
+        if s % 2 == 0:
+            _jacobi_sweep(A, B)
+        else:
+            _jacobi_sweep(B, A)
+
+
+def jacobi_2d_kernel(TSTEPS, A, B):
+    """jacobi_2d_numpy.kernel: ``TSTEPS - 1`` ping-pong iterations in place."""
+    _f64(A, B)
+    for _ in range(1, TSTEPS):
+        _jacobi_sweep(A, B)
+        _jacobi_sweep(B, A)
+
+
+# --------------------------------------------------------------------------
+# 3-D 7-point heat
+# --------------------------------------------------------------------------
+def _heat_sweep(src, dst):
+    """One explicit heat sweep on the interior of a cubic grid.
+
+    Follows heat_3d_numpy.py:7-13: per axis ``0.125 * ((x[+1] - 2.0*c) + x[-1])``,
+    the three axis terms summed i, j, k left to right, then the centre added.
+    """
+    c = src[1:-1, 1:-1, 1:-1]
+    c2 = 2.0 * c
+    ti = 0.125 * ((src[2:, 1:-1, 1:-1] - c2) + src[:-2, 1:-1, 1:-1])
+    tj = 0.125 * ((src[1:-1, 2:, 1:-1] - c2) + src[1:-1, :-2, 1:-1])
+    tk = 0.125 * ((src[1:-1, 1:-1, 2:] - c2) + src[1:-1, 1:-1, :-2])
+    dst[1:-1, 1:-1, 1:-1] = ((ti + tj) + tk) + c
+
+
+def heat_sweeps(A, B, nsweeps):
+    """Run ``nsweeps`` alternating heat sweeps: odd A->B, even B->A."""
+    _f64(A, B)
+    for s in range(nsweeps):
+        if s % 2 == 0:
+            _heat_sweep(A, B)
+        else:
+            _heat_sweep(B, A)
+
+
+def heat_3d_kernel(TSTEPS, A, B):
+    """heat_3d_numpy.kernel: ``TSTEPS - 1`` ping-pong iterations in place."""
+    _f64(A, B)
+    for _ in range(1, TSTEPS):
+        _heat_sweep(A, B)
+        _heat_sweep(B, A)
+
+
+# --------------------------------------------------------------------------
+# Horizontal diffusion with flux limiter
+# --------------------------------------------------------------------------
+def _limit(r, grad):
+    """Flux limiter of hdiff_numpy.py:12-17/20-25: zero where r*grad > 0."""
+    return np.where(r * grad > 0, 0, r)
+
+
+def hdiff(in_field, out_field, coeff):
+    """hdiff_numpy.hdiff: one application, ``out_field`` written in place.
+
+    Shapes: in_field (I+4, J+4, K), out_field and coeff (I, J, K).
+    lap is defined on (I+2, J+2), flx on (I+1, J), fly on (I, J+1)
+    (hdiff_numpy.py:7-25); the limiter zeroes a flux whose product with the
+    local gradient is strictly positive.
+    """
+    _f64(in_field, out_field, coeff)
+    I, J, K = out_field.shape
+    f = in_field
+    # hdiff_numpy.py:7-9 -- 4*c - (((i+1) + (i-1)) + (j+1)) + (j-1))
+    nb = f[2:I + 4, 1:J + 3] + f[0:I + 2, 1:J + 3]
+    nb = nb + f[1:I + 3, 2:J + 4]
+    nb = nb + f[1:I + 3, 0:J + 2]
+    lap = 4.0 * f[1:I + 3, 1:J + 3] - nb
+
+    # hdiff_numpy.py:11-17
+    r = lap[1:, 1:J + 1] - lap[:-1, 1:J + 1]
+    g = f[2:I + 3, 2:J + 2] - f[1:I + 2, 2:J + 2]
+    flx = _limit(r, g)
+
+    # hdiff_numpy.py:19-25
+    r = lap[1:I + 1, 1:] - lap[1:I + 1, :-1]
+    g = f[2:I + 2, 2:J + 3] - f[2:I + 2, 1:J + 2]
+    fly = _limit(r, g)
+
+    # hdiff_numpy.py:27-29 -- ((dflx_i + fly[j+1]) - fly[j])
+    div = flx[1:] - flx[:-1]
+    div = div + fly[:, 1:]
+    div = div - fly[:, :-1]
+    out_field[:, :, :] = f[2:I + 2, 2:J + 2] - coeff * div
+
+
+def hdiff_iterated(in_field, out_field, coeff, nsteps):
+    """Build-defined iteration of hdiff (the reference has no time loop).
+
+    Step n feeds the previous output back into the interior of a copy of
+    ``in_field`` whose 2-cell halo stays fixed at its original values; the
+    final output lands in ``out_field``.  ``nsteps == 1`` is exactly the
+    reference hdiff.  ``in_field`` is not modified.
+    """
+    if nsteps < 1:
+        raise ValueError("nsteps must be >= 1")
+    I, J, _ = out_field.shape
+    work = np.array(in_field, dtype=np.float64, copy=True)
+    res = np.empty(out_field.shape, dtype=np.float64)
+    for n in range(nsteps):
+        hdiff(work, res, coeff)
+        if n + 1 < nsteps:
+            work[2:I + 2, 2:J + 2] = res
+    out_field[:, :, :] = res
+
+
+# --------------------------------------------------------------------------
+# Implicit vertical advection (Thomas solve per (i, j) column)
+# --------------------------------------------------------------------------
+def vadv(utens_stage, u_stage, wcon, u_pos, utens, dtr_stage):
+    """vadv_numpy.vadv: per-column tridiagonal solve, ``utens_stage`` in place.
+
+    For column (i, j) and level k, with s_k = wcon[i+1,j,k] + wcon[i,j,k]:
+      a_k = (-0.25 * s_k) * BET_P,  c_k = (0.25 * s_{k+1}) * BET_P,
+      b_k = dtr - a_k - c_k (a_k absent at k = 0, c_k absent at k = K-1),
+      d_k = dtr*u_pos + utens + utens_stage + corr_k, with the explicit
+      BET_M correction ``-as*(u[k-1]-u[k]) - cs*(u[k+1]-u[k])``.
+    Forward sweep takes the reciprocal of the pivot then multiplies
+    (vadv_numpy.py:28-30, 51-53, 67-68); back substitution writes
+    ``dtr * (x_k - u_pos_k)`` (vadv_numpy.py:70-78).  Requires K >= 2.
+    """
+    _f64(utens_stage, u_stage, wcon, u_pos, utens)
+    I, J, K = utens_stage.shape
+    if K < 2:
+        raise ValueError("vadv needs K >= 2")
+    ccol = np.empty((I, J, K), dtype=np.float64)
+    dcol = np.empty((I, J, K), dtype=np.float64)
+
+    def wsum(k):
+        return wcon[1:, :, k] + wcon[:-1, :, k]
+
+    def rhs(k, corr):
+        return ((dtr_stage * u_pos[:, :, k] + utens[:, :, k]) +
+                utens_stage[:, :, k]) + corr
+
+    # k = 0 (vadv_numpy.py:15-30)
+    gcv = 0.25 * wsum(1)
+    cs = gcv * BET_M
+    c0 = gcv * BET_P
+    bcol = dtr_stage - c0
+    corr = -cs * (u_stage[:, :, 1] - u_stage[:, :, 0])
+    d0 = rhs(0, corr)
+    inv = 1.0 / bcol
+    ccol[:, :, 0] = c0 * inv
+    dcol[:, :, 0] = d0 * inv
+
+    # 0 < k < K-1 (vadv_numpy.py:32-53)
+    for k in range(1, K - 1):
+        gav = -0.25 * wsum(k)
+        gcv = 0.25 * wsum(k + 1)
+        as_ = gav * BET_M
+        cs = gcv * BET_M
+        acol = gav * BET_P
+        ck = gcv * BET_P
+        bcol = dtr_stage - acol - ck
+        corr = (-as_ * (u_stage[:, :, k - 1] - u_stage[:, :, k]) -
+                cs * (u_stage[:, :, k + 1] - u_stage[:, :, k]))
+        dk = rhs(k, corr)
+        inv = 1.0 / (bcol - ccol[:, :, k - 1] * acol)
+        ccol[:, :, k] = ck * inv
+        dcol[:, :, k] = (dk - dcol[:, :, k - 1] * acol) * inv
+
+    # k = K-1 (vadv_numpy.py:55-68)
+    k = K - 1
+    gav = -0.25 * wsum(k)
+    as_ = gav * BET_M
+    acol = gav * BET_P
+    bcol = dtr_stage - acol
+    corr = -as_ * (u_stage[:, :, k - 1] - u_stage[:, :, k])
+    dk = rhs(k, corr)
+    inv = 1.0 / (bcol - ccol[:, :, k - 1] * acol)
+    dcol[:, :, k] = (dk - dcol[:, :, k - 1] * acol) * inv
+
+    # back substitution (vadv_numpy.py:70-78)
+    x = dcol[:, :, K - 1].copy()
+    utens_stage[:, :, K - 1] = dtr_stage * (x - u_pos[:, :, K - 1])
+    for k in range(K - 2, -1, -1):
+        x = dcol[:, :, k] - ccol[:, :, k] * x
+        utens_stage[:, :, k] = dtr_stage * (x - u_pos[:, :, k])

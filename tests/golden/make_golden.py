@@ -1,0 +1,83 @@
+"""Writes the golden vectors under tests/golden/ (run from the repo root:
+``python tests/golden/make_golden.py``).
+
+The reference holds no golden vectors, fixtures or known-answer tests for
+this path, and none of its code is imported here: the expected outputs come
+from the scalar brute-force restatement ``oracle.scalar`` (pure-Python point
+loops), which is itself pinned by the closed-form tests in
+tests/test_oracle.py.  The vectorised oracle and the GPU kernels are checked
+against these files bit for bit.  Inputs are stored, not only seeds, because
+numpy's Generator stream is not guaranteed across versions.
+
+Cases (all FP64):
+  jacobi_2d   N=6  TSTEPS=4, reference init (fromfunction) and seeded uniform
+  heat_3d     N=6  TSTEPS=3, reference init and seeded uniform (B = A copy and
+              independent B)
+  hdiff       (I,J,K) = (3,4,5) and (5,3,2), reference init and config init
+  hdiff_it    (4,4,3) with nsteps = 3 (build-defined iteration)
+  vadv        (2,3,6) reference init (wcon in [0,1)) and config init
+              (wcon in [-1,1)), and the K=2 edge case
+"""
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import scalar  # noqa: E402
+from npbench_amd.benchmarks.polybench.jacobi_2d import jacobi_2d  # noqa: E402
+from npbench_amd.benchmarks.polybench.heat_3d import heat_3d  # noqa: E402
+from npbench_amd.benchmarks.weather_stencils.hdiff import hdiff  # noqa: E402
+from npbench_amd.benchmarks.weather_stencils.vadv import vadv  # noqa: E402
+
+
+def save(name, **arrays):
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **arrays)
+    print(f"wrote {os.path.relpath(path, ROOT)} ({os.path.getsize(path)} bytes)")
+
+
+def main():
+    # Jacobi 2-D
+    for tag, (A, B) in (("ref", jacobi_2d.initialize(6)),
+                        ("uniform", jacobi_2d.initialize_uniform(6, seed=7))):
+        oa, ob = scalar.jacobi_2d_kernel(4, A, B)
+        save(f"jacobi_2d_{tag}", TSTEPS=4, A=A, B=B, A_out=oa, B_out=ob)
+
+    # heat 3-D
+    A, B = heat_3d.initialize(6)
+    oa, ob = scalar.heat_3d_kernel(3, A, B)
+    save("heat_3d_ref", TSTEPS=3, A=A, B=B, A_out=oa, B_out=ob)
+    rng = np.random.default_rng(11)
+    A = rng.random((6, 6, 6))
+    B = rng.random((6, 6, 6))
+    oa, ob = scalar.heat_3d_kernel(3, A, B)
+    save("heat_3d_uniform", TSTEPS=3, A=A, B=B, A_out=oa, B_out=ob)
+
+    # hdiff
+    for tag, shape, init in (("ref", (3, 4, 5), hdiff.initialize),
+                             ("config", (5, 3, 2), hdiff.initialize_config)):
+        in_field, out_field, coeff = init(*shape)
+        out = scalar.hdiff(in_field, coeff)
+        save(f"hdiff_{tag}", in_field=in_field, out_field=out_field, coeff=coeff, out=out)
+    in_field, out_field, coeff = hdiff.initialize_config(4, 4, 3, seed=3)
+    out = scalar.hdiff_iterated(in_field, coeff, 3)
+    save("hdiff_iterated3", nsteps=3, in_field=in_field, out_field=out_field, coeff=coeff,
+         out=out)
+
+    # vadv
+    for tag, shape, init in (("ref", (2, 3, 6), vadv.initialize),
+                             ("config", (2, 3, 6), vadv.initialize_config),
+                             ("k2", (3, 2, 2), vadv.initialize_config)):
+        dtr, us, u, w, up, ut = init(*shape)
+        out = scalar.vadv(us, u, w, up, ut, dtr)
+        save(f"vadv_{tag}", dtr_stage=dtr, utens_stage=us, u_stage=u, wcon=w, u_pos=up,
+             utens=ut, out=out)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,84 @@
+"""The C ABI library loads and exports what include/npbx.h declares (CPU only).
+
+No compute runs here: the calls below are rejected by argument checks that
+happen before any HIP call.
+"""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from npbench_amd import _native as nat
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include",
+                      "npbx.h")
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    out = {}
+    for m in re.finditer(r"\bint\s+(npbx_\w+)\s*\(([^)]*)\)\s*;", text):
+        params = [p for p in m.group(2).split(",") if p.strip() and p.strip() != "void"]
+        out[m.group(1)] = len(params)
+    return out
+
+
+def test_header_declares_entry_points():
+    decl = declared()
+    for name in ("npbx_jacobi2d", "npbx_heat3d", "npbx_hdiff", "npbx_vadv", "npbx_sync",
+                 "npbx_cast", "npbx_last_error"):
+        assert name in decl
+
+
+def test_every_declared_symbol_is_exported_and_bound():
+    lib = nat.lib()
+    decl = declared()
+    for name, nparams in decl.items():
+        assert hasattr(lib, name), f"{name} not exported by libnpbx.so"
+        assert name in nat.SIGNATURES, f"{name} has no ctypes prototype"
+        assert len(nat.SIGNATURES[name]) == nparams, f"{name}: arity mismatch"
+    assert set(nat.SIGNATURES) == set(decl), "prototypes for undeclared symbols"
+
+
+def test_version_string():
+    assert "gfx950" in nat.version()
+
+
+@pytest.mark.parametrize("call,needle", [
+    (lambda: nat.call("npbx_jacobi2d", 0, None, None, -5, 3, 1, None), "N=-5"),
+    (lambda: nat.call("npbx_heat3d_sweeps", 0, None, None, 10, -1, 2, 1, None), "negative"),
+    (lambda: nat.call("npbx_hdiff", 0, None, None, None, 4, 4, 4, 0, None), "nsteps"),
+    (lambda: nat.call("npbx_vadv", 0, None, None, None, None, None, 0.15, 2, 2, 1, None), "K >= 2"),
+    (lambda: nat.call("npbx_cast", 0, None, 7, None, 10, None), "null"),
+    (lambda: nat.call("npbx_jacobi2d", 9, 1, 2, 10, 3, 1, None), "dtype"),
+])
+def test_argument_errors_raise_with_message(call, needle):
+    with pytest.raises(nat.NpbxError) as ei:
+        call()
+    assert ei.value.code == nat.EINVAL
+    assert needle in str(ei.value)
+
+
+def test_empty_work_is_a_no_op():
+    # nothing to update: no launch, no device needed
+    assert nat.call("npbx_jacobi2d", 0, None, None, 2, 10, 1, None) == nat.OK
+    assert nat.call("npbx_heat3d", 0, None, None, 10, 1, 1, None) == nat.OK
+    assert nat.call("npbx_cast", 0, None, 1, None, 0, None) == nat.OK
+
+
+def test_last_error_buffer_is_terminated():
+    with pytest.raises(nat.NpbxError):
+        nat.call("npbx_hdiff", 0, None, None, None, -1, 1, 1, 1, None)
+    buf = ctypes.create_string_buffer(8)
+    assert nat.lib().npbx_last_error(buf, len(buf)) == nat.OK
+    assert len(buf.value) == 7
+
+
+def test_missing_library_is_loud(monkeypatch, tmp_path):
+    monkeypatch.setattr(nat, "_lib", None)
+    monkeypatch.setattr(nat, "LIB_PATH", tmp_path / "libnpbx.so")
+    with pytest.raises(nat.NativeLibraryMissing):
+        nat.lib()

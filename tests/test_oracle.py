@@ -1,0 +1,480 @@
+"""Pins the FP64 oracle (CPU only).
+
+The reference has no golden vectors for this path, so each oracle function is
+pinned by what the mathematics fixes, chosen so that a plausible mistake (a
+dropped term, a wrong weight, a swapped index or sign, an off-by-one slice)
+fails at least one check:
+
+  jacobi_2d  affine fixed point; discrete Fourier eigenmode with its exact
+             eigenvalue 0.2*(1 + 2cos a + 2cos b); untouched boundaries;
+             bitwise against the scalar brute force and the golden vectors.
+  heat_3d    affine fixed point; eigenmode 1 + 0.25*(cos a + cos b + cos c - 3);
+             the quadratic i^2 gains exactly 0.25 per sweep inside the cone
+             the fixed boundary has not reached (all arithmetic exact);
+             brute force and golden vectors.
+  hdiff      integer linear and quadratic inputs give out == centre exactly;
+             coeff = 0 gives the centre; a single spike gives the limiter's
+             closed form (c, -c, 1, -c, c) and not the unlimited 1 + 6c;
+             x^4 gives in + 24*coeff; brute force and golden vectors.
+  vadv       an exact solution built from the Crank-Nicolson structure
+             (equal implicit/explicit weights) for arbitrary wcon; wcon = 0
+             gives utens + utens_stage; the elimination matches a banded LU
+             solve (scipy.linalg.solve_banded); linearity; brute force and
+             golden vectors (including K = 2).
+
+The last section plants mistakes in the oracle (monkeypatched copies) and
+checks the pins above catch them.
+"""
+
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from oracle import scalar, stencils as S
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+# ------------------------------------------------------------------ helpers
+def jacobi_affine_ok(kernel=S.jacobi_2d_kernel, n=20, tsteps=4):
+    i, j = np.indices((n, n), dtype=np.float64)
+    A = 0.5 + 0.25 * i - 0.125 * j
+    B = A.copy()
+    a, b = A.copy(), B.copy()
+    kernel(tsteps, a, b)
+    return np.allclose(a, A, rtol=0, atol=1e-13) and np.allclose(b, B, rtol=0, atol=1e-13)
+
+
+def jacobi_mode_err(kernel=S.jacobi_2d_kernel, n=33, p=2, q=3, tsteps=4):
+    i, j = np.indices((n, n), dtype=np.float64)
+    al, be = p * np.pi / (n - 1), q * np.pi / (n - 1)
+    mode = np.sin(al * i) * np.sin(be * j)
+    lam = 0.2 * (1.0 + 2.0 * np.cos(al) + 2.0 * np.cos(be))
+    A = mode.copy()
+    B = mode.copy()  # boundary of B is the mode's boundary (zero)
+    kernel(tsteps, A, B)
+    s = 2 * (tsteps - 1)
+    return max(np.max(np.abs(A - lam ** s * mode)), np.max(np.abs(B - lam ** (s - 1) * mode)))
+
+
+def heat_mode_err(kernel=S.heat_3d_kernel, n=17, pqr=(1, 2, 3), tsteps=3):
+    idx = np.indices((n, n, n), dtype=np.float64)
+    angles = [m * np.pi / (n - 1) for m in pqr]
+    mode = np.ones((n, n, n))
+    for ax, a in zip(idx, angles):
+        mode = mode * np.sin(a * ax)
+    lam = 1.0 + 0.25 * sum(np.cos(a) - 1.0 for a in angles)
+    A = mode.copy()
+    B = mode.copy()
+    kernel(tsteps, A, B)
+    s = 2 * (tsteps - 1)
+    return max(np.max(np.abs(A - lam ** s * mode)), np.max(np.abs(B - lam ** (s - 1) * mode)))
+
+
+def heat_cone_ok(kernel=S.heat_3d_kernel, n=12, tsteps=3):
+    i = np.indices((n, n, n))[0].astype(np.float64)
+    A = i * i
+    B = A.copy()
+    a, b = A.copy(), B.copy()
+    kernel(tsteps, a, b)
+    sweeps = 2 * (tsteps - 1)
+    idx = np.indices((n, n, n))
+    dist = np.minimum(idx, n - 1 - idx).min(axis=0)
+    mask = dist >= sweeps
+    assert mask.any()
+    return np.array_equal(a[mask], (A + 0.25 * sweeps)[mask])
+
+
+def hdiff_spike_ok(fn=S.hdiff, I=9, J=4, K=2, c=0.125):
+    in_field = np.zeros((I + 4, J + 4, K))
+    p0 = 2 + I // 2  # in-field row of the spike, away from the halo
+    in_field[p0, :, :] = 1.0  # depends on i only: a spike along i
+    coeff = np.full((I, J, K), c)
+    out = np.empty((I, J, K))
+    fn(in_field, out, coeff)
+    want = np.zeros((I, J, K))
+    x0 = p0 - 2
+    want[x0] = 1.0                     # limited: both fluxes at the peak are zeroed
+    want[x0 - 1] = want[x0 + 1] = -c
+    want[x0 - 2] = want[x0 + 2] = c
+    return np.array_equal(out, want)
+
+
+def vadv_exact_err(fn=S.vadv, I=4, J=5, K=12, wlo=-1.0, seed=0):
+    rng = np.random.default_rng(seed)
+    dtr = 0.15
+    X = rng.random((I, J, K))
+    wcon = (1.0 - wlo) * rng.random((I + 1, J, K)) + wlo
+    u_stage = 0.7 - X
+    u_pos = rng.random((I, J, K))
+    utens = rng.random((I, J, K))
+    utens_stage = dtr * X - dtr * u_pos - utens
+    out = utens_stage.copy()
+    fn(out, u_stage, wcon, u_pos, utens, dtr)
+    want = dtr * (X - u_pos)
+    return np.max(np.abs(out - want)) / np.max(np.abs(want))
+
+
+# ------------------------------------------------------------------ Jacobi
+def test_jacobi_affine_fixed_point():
+    assert jacobi_affine_ok()
+
+
+@pytest.mark.parametrize("p,q", [(1, 1), (2, 3), (5, 1)])
+def test_jacobi_eigenmode(p, q):
+    assert jacobi_mode_err(p=p, q=q) < 1e-13
+
+
+def test_jacobi_boundaries_untouched():
+    rng = np.random.default_rng(0)
+    A, B = rng.random((15, 15)), rng.random((15, 15))
+    a, b = A.copy(), B.copy()
+    S.jacobi_2d_kernel(5, a, b)
+    for x, y in ((a, A), (b, B)):
+        assert np.array_equal(x[0], y[0]) and np.array_equal(x[-1], y[-1])
+        assert np.array_equal(x[:, 0], y[:, 0]) and np.array_equal(x[:, -1], y[:, -1])
+
+
+@pytest.mark.parametrize("n,tsteps", [(3, 3), (4, 1), (5, 2), (8, 4)])
+def test_jacobi_bruteforce_bitwise(n, tsteps):
+    rng = np.random.default_rng(n + tsteps)
+    A, B = rng.random((n, n)), rng.random((n, n))
+    a, b = A.copy(), B.copy()
+    S.jacobi_2d_kernel(tsteps, a, b)
+    ra, rb = scalar.jacobi_2d_kernel(tsteps, A, B)
+    assert np.array_equal(a, ra) and np.array_equal(b, rb)
+
+
+def test_jacobi_transpose_symmetry():
+    rng = np.random.default_rng(3)
+    A, B = rng.random((12, 12)), rng.random((12, 12))
+    a, b = A.copy(), B.copy()
+    S.jacobi_2d_kernel(4, a, b)
+    at, bt = A.T.copy(), B.T.copy()
+    S.jacobi_2d_kernel(4, at, bt)
+    assert np.allclose(a, at.T, rtol=1e-14, atol=0)
+
+
+# ------------------------------------------------------------------ heat
+def test_heat_affine_fixed_point():
+    i, j, k = np.indices((10, 10, 10), dtype=np.float64)
+    A = 1.0 + 0.5 * i - 0.25 * j + 0.125 * k
+    a, b = A.copy(), A.copy()
+    S.heat_3d_kernel(4, a, b)
+    assert np.array_equal(a, A) and np.array_equal(b, A)
+
+
+@pytest.mark.parametrize("pqr", [(1, 1, 1), (1, 2, 3), (4, 1, 2)])
+def test_heat_eigenmode(pqr):
+    assert heat_mode_err(pqr=pqr) < 1e-13
+
+
+def test_heat_quadratic_cone_exact():
+    assert heat_cone_ok()
+
+
+@pytest.mark.parametrize("n,tsteps", [(3, 2), (4, 3), (6, 2)])
+def test_heat_bruteforce_bitwise(n, tsteps):
+    rng = np.random.default_rng(n * tsteps)
+    A, B = rng.random((n, n, n)), rng.random((n, n, n))
+    a, b = A.copy(), B.copy()
+    S.heat_3d_kernel(tsteps, a, b)
+    ra, rb = scalar.heat_3d_kernel(tsteps, A, B)
+    assert np.array_equal(a, ra) and np.array_equal(b, rb)
+
+
+# ------------------------------------------------------------------ hdiff
+def _grid_fn(I, J, K, f):
+    p, q, k = np.indices((I + 4, J + 4, K), dtype=np.float64)
+    return f(p, q, k)
+
+
+@pytest.mark.parametrize("f", [lambda p, q, k: 3 * p - 2 * q + k,
+                               lambda p, q, k: p * p + q * q - k,
+                               lambda p, q, k: p * q + 5.0])
+def test_hdiff_polynomial_inputs_keep_centre(f):
+    I, J, K = 6, 5, 3
+    in_field = _grid_fn(I, J, K, f)
+    coeff = np.random.default_rng(1).random((I, J, K))
+    out = np.empty((I, J, K))
+    S.hdiff(in_field, out, coeff)
+    assert np.array_equal(out, in_field[2:-2, 2:-2])
+
+
+def test_hdiff_zero_coeff_is_centre():
+    rng = np.random.default_rng(2)
+    in_field = rng.random((9, 8, 4))
+    out = np.empty((5, 4, 4))
+    S.hdiff(in_field, out, np.zeros((5, 4, 4)))
+    assert np.array_equal(out, in_field[2:-2, 2:-2])
+
+
+def test_hdiff_limiter_spike():
+    assert hdiff_spike_ok()
+
+
+def test_hdiff_quartic_closed_form():
+    I, J, K = 7, 3, 2
+    in_field = _grid_fn(I, J, K, lambda p, q, k: (p - 4.0) ** 4)
+    coeff = np.random.default_rng(4).random((I, J, K))
+    out = np.empty((I, J, K))
+    S.hdiff(in_field, out, coeff)
+    # lap = -(12 x^2 + 2): flux r = -12(2x+1) opposes the gradient everywhere,
+    # so nothing is limited and the flux divergence is -24.
+    want = in_field[2:-2, 2:-2] + 24.0 * coeff
+    assert np.allclose(out, want, rtol=1e-15, atol=1e-12)
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (3, 4, 5), (5, 2, 3)])
+def test_hdiff_bruteforce_bitwise(shape):
+    I, J, K = shape
+    rng = np.random.default_rng(sum(shape))
+    in_field, coeff = rng.random((I + 4, J + 4, K)), rng.random((I, J, K))
+    out = np.empty((I, J, K))
+    S.hdiff(in_field, out, coeff)
+    assert np.array_equal(out, scalar.hdiff(in_field, coeff))
+
+
+def test_hdiff_iterated_one_step_is_reference():
+    rng = np.random.default_rng(8)
+    in_field, coeff = rng.random((8, 9, 3)), rng.random((4, 5, 3))
+    a, b = np.empty((4, 5, 3)), np.empty((4, 5, 3))
+    S.hdiff(in_field, a, coeff)
+    S.hdiff_iterated(in_field, b, coeff, 1)
+    assert np.array_equal(a, b)
+    c = np.empty((4, 5, 3))
+    S.hdiff_iterated(in_field, c, coeff, 3)
+    assert np.array_equal(c, scalar.hdiff_iterated(in_field, coeff, 3))
+
+
+# ------------------------------------------------------------------ vadv
+@pytest.mark.parametrize("wlo", [0.0, -1.0])
+def test_vadv_exact_solution(wlo):
+    assert vadv_exact_err(wlo=wlo) < 1e-12
+
+
+def test_vadv_zero_wind():
+    rng = np.random.default_rng(5)
+    I, J, K = 3, 4, 9
+    f = [rng.random((I, J, K)) for _ in range(4)]
+    us, u, up, ut = f
+    out = us.copy()
+    S.vadv(out, u, np.zeros((I + 1, J, K)), up, ut, 0.15)
+    assert np.allclose(out, ut + us, rtol=1e-14, atol=1e-14)
+
+
+def test_vadv_matches_banded_solver():
+    """The elimination against LAPACK's banded solve of the same system.
+
+    The tridiagonal system is assembled here from the reference's
+    coefficient definitions (vadv_numpy.py:15-68); the exact-solution test
+    above pins that assembly independently of this restatement.
+    """
+    la = pytest.importorskip("scipy.linalg")
+    rng = np.random.default_rng(6)
+    I, J, K, dtr = 2, 3, 10, 0.15
+    us, u, up, ut = (rng.random((I, J, K)) for _ in range(4))
+    w = rng.random((I + 1, J, K))
+    out = us.copy()
+    S.vadv(out, u, w, up, ut, dtr)
+    for i in range(I):
+        for j in range(J):
+            s = w[i + 1, j] + w[i, j]
+            a = np.zeros(K)
+            c = np.zeros(K)
+            a[1:] = -0.25 * s[1:] * 0.5
+            c[:-1] = 0.25 * s[1:] * 0.5
+            b = dtr - a - c
+            uu = u[i, j]
+            corr = np.zeros(K)
+            corr[1:] += -a[1:] * (uu[:-1] - uu[1:])
+            corr[:-1] += -c[:-1] * (uu[1:] - uu[:-1])
+            d = dtr * up[i, j] + ut[i, j] + us[i, j] + corr
+            ab = np.zeros((3, K))
+            ab[0, 1:] = c[:-1]
+            ab[1] = b
+            ab[2, :-1] = a[1:]
+            x = la.solve_banded((1, 1), ab, d)
+            np.testing.assert_allclose(out[i, j], dtr * (x - up[i, j]), rtol=1e-11, atol=1e-12)
+
+
+def test_vadv_linearity():
+    rng = np.random.default_rng(7)
+    I, J, K = 3, 3, 8
+    w = rng.random((I + 1, J, K))
+
+    def run(fields):
+        out = fields[0].copy()
+        S.vadv(out, fields[1], w, fields[2], fields[3], 0.15)
+        return out
+
+    p = [rng.random((I, J, K)) for _ in range(4)]
+    q = [rng.random((I, J, K)) for _ in range(4)]
+    comb = [2.0 * x - 3.0 * y for x, y in zip(p, q)]
+    assert np.allclose(run(comb), 2.0 * run(p) - 3.0 * run(q), rtol=1e-12, atol=1e-11)
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 2), (2, 3, 3), (3, 2, 7)])
+def test_vadv_bruteforce_bitwise(shape):
+    I, J, K = shape
+    rng = np.random.default_rng(K)
+    us, u, up, ut = (rng.random((I, J, K)) for _ in range(4))
+    w = 2.0 * rng.random((I + 1, J, K)) - 1.0
+    out = us.copy()
+    S.vadv(out, u, w, up, ut, 0.15)
+    assert np.array_equal(out, scalar.vadv(us, u, w, up, ut, 0.15))
+
+
+def test_vadv_rejects_k1():
+    z = np.zeros((1, 1, 1))
+    with pytest.raises(ValueError):
+        S.vadv(z.copy(), z, np.zeros((2, 1, 1)), z, z, 0.15)
+
+
+# ------------------------------------------------------------------ golden
+def _golden(name):
+    with np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def test_golden_files_present():
+    names = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+    assert len(names) >= 10, names
+
+
+@pytest.mark.parametrize("name", ["jacobi_2d_ref", "jacobi_2d_uniform"])
+def test_golden_jacobi(name):
+    g = _golden(name)
+    a, b = g["A"].copy(), g["B"].copy()
+    S.jacobi_2d_kernel(int(g["TSTEPS"]), a, b)
+    assert np.array_equal(a, g["A_out"]) and np.array_equal(b, g["B_out"])
+
+
+@pytest.mark.parametrize("name", ["heat_3d_ref", "heat_3d_uniform"])
+def test_golden_heat(name):
+    g = _golden(name)
+    a, b = g["A"].copy(), g["B"].copy()
+    S.heat_3d_kernel(int(g["TSTEPS"]), a, b)
+    assert np.array_equal(a, g["A_out"]) and np.array_equal(b, g["B_out"])
+
+
+@pytest.mark.parametrize("name", ["hdiff_ref", "hdiff_config", "hdiff_iterated3"])
+def test_golden_hdiff(name):
+    g = _golden(name)
+    out = g["out_field"].copy()
+    S.hdiff_iterated(g["in_field"], out, g["coeff"], int(g.get("nsteps", 1)))
+    assert np.array_equal(out, g["out"])
+
+
+@pytest.mark.parametrize("name", ["vadv_ref", "vadv_config", "vadv_k2"])
+def test_golden_vadv(name):
+    g = _golden(name)
+    out = g["utens_stage"].copy()
+    S.vadv(out, g["u_stage"], g["wcon"], g["u_pos"], g["utens"], float(g["dtr_stage"]))
+    assert np.array_equal(out, g["out"])
+
+
+# ------------------------------------------------------------------ planted mistakes
+def _mutant_jacobi_sweep(weight=0.2, drop=None, shift=False):
+    def sweep(src, dst):
+        c = src[1:-1, 1:-1]
+        terms = [src[1:-1, :-2], src[1:-1, 2:], src[2:, 1:-1],
+                 src[:-2, 1:-1] if not shift else src[2:, 1:-1]]
+        acc = c
+        for n, t in enumerate(terms):
+            if n != drop:
+                acc = acc + t
+        dst[1:-1, 1:-1] = weight * acc
+    return sweep
+
+
+@pytest.mark.parametrize("mutant", [dict(weight=0.25), dict(drop=3), dict(shift=True)])
+def test_planted_jacobi_mistakes_are_caught(monkeypatch, mutant):
+    monkeypatch.setattr(S, "_jacobi_sweep", _mutant_jacobi_sweep(**mutant))
+    caught = (not jacobi_affine_ok()) or jacobi_mode_err() > 1e-13
+    assert caught
+
+
+def _mutant_heat_sweep(kind):
+    def sweep(src, dst):
+        c = src[1:-1, 1:-1, 1:-1]
+        c2 = 2.0 * c
+        ti = 0.125 * ((src[2:, 1:-1, 1:-1] - c2) + src[:-2, 1:-1, 1:-1])
+        tj = 0.125 * ((src[1:-1, 2:, 1:-1] - c2) + src[1:-1, :-2, 1:-1])
+        km = src[1:-1, 1:-1, :-2] if kind != "k-index" else src[1:-1, 1:-1, 2:]
+        tk = 0.125 * ((src[1:-1, 1:-1, 2:] - c2) + km)
+        if kind == "drop-k":
+            tk = 0.0 * tk
+        if kind == "no-centre":
+            dst[1:-1, 1:-1, 1:-1] = (ti + tj) + tk
+        elif kind == "weight":
+            dst[1:-1, 1:-1, 1:-1] = ((ti + tj) + 1.25 * tk) + c
+        else:
+            dst[1:-1, 1:-1, 1:-1] = ((ti + tj) + tk) + c
+    return sweep
+
+
+@pytest.mark.parametrize("kind", ["drop-k", "no-centre", "weight", "k-index"])
+def test_planted_heat_mistakes_are_caught(monkeypatch, kind):
+    monkeypatch.setattr(S, "_heat_sweep", _mutant_heat_sweep(kind))
+    caught = heat_mode_err() > 1e-13 or not heat_cone_ok()
+    assert caught
+
+
+@pytest.mark.parametrize("limiter", [
+    lambda r, g: r,                               # limiter dropped
+    lambda r, g: np.where(r * g >= 0, 0, r),      # '>=' for '>'
+    lambda r, g: np.where(r * g < 0, 0, r),       # sign of the test flipped
+])
+def test_planted_hdiff_limiter_mistakes_are_caught(monkeypatch, limiter):
+    monkeypatch.setattr(S, "_limit", limiter)
+    assert not hdiff_spike_ok()
+
+
+def _mutant_hdiff(mistake):
+    """A copy of the oracle's hdiff with one plausible slicing mistake."""
+    def fn(in_field, out_field, coeff):
+        I, J, K = out_field.shape
+        f = in_field
+        nb = f[2:I + 4, 1:J + 3] + f[0:I + 2, 1:J + 3]
+        nb = nb + f[1:I + 3, 2:J + 4]
+        nb = nb + f[1:I + 3, 0:J + 2]
+        lap = 4.0 * f[1:I + 3, 1:J + 3] - nb
+        r = lap[1:, 1:J + 1] - lap[:-1, 1:J + 1]
+        g = f[2:I + 3, 2:J + 2] - f[1:I + 2, 2:J + 2]
+        flx = S._limit(r, g)
+        r = lap[1:I + 1, 1:] - lap[1:I + 1, :-1]
+        g = f[2:I + 2, 2:J + 3] - f[2:I + 2, 1:J + 2]
+        fly = S._limit(r, g)
+        if mistake == "flx-order":
+            div = flx[:-1] - flx[1:]
+        else:
+            div = flx[1:] - flx[:-1]
+        div = div + fly[:, 1:]
+        div = div - fly[:, :-1]
+        centre = f[2:I + 2, 3:J + 3] if mistake == "centre-j" else f[2:I + 2, 2:J + 2]
+        out_field[:, :, :] = centre - coeff * div
+    return fn
+
+
+@pytest.mark.parametrize("mistake", ["flx-order", "centre-j"])
+def test_planted_hdiff_slice_mistakes_are_caught(mistake):
+    fn = _mutant_hdiff(mistake)
+    I, J, K = 6, 5, 3
+    in_field = _grid_fn(I, J, K, lambda p, q, k: 3 * p - 2 * q + k)
+    out = np.empty((I, J, K))
+    fn(in_field, out, np.random.default_rng(1).random((I, J, K)))
+    caught = (not hdiff_spike_ok(fn=fn)) or not np.array_equal(out, in_field[2:-2, 2:-2])
+    assert caught
+
+
+def test_planted_vadv_mistakes_are_caught(monkeypatch):
+    # a wrong implicit weight breaks the exact solution
+    monkeypatch.setattr(S, "BET_P", 0.6)
+    assert vadv_exact_err() > 1e-6
+    monkeypatch.setattr(S, "BET_P", 0.5)
+    # an explicit correction with the wrong sign breaks it as well
+    monkeypatch.setattr(S, "BET_M", -0.5)
+    assert vadv_exact_err() > 1e-6
