@@ -87,6 +87,11 @@ int npbx_event_elapsed_ms(void *start, void *stop, float *ms);
 int npbx_cast(int src_dtype, const void *src, int dst_dtype, void *dst,
               int64_t count, void *stream);
 
+/* Streaming device copy of `bytes` (a multiple of 16, 16-byte aligned
+ * pointers).  No reference counterpart: calibrates the achievable HBM rate
+ * and the HBM traffic counters that the stencil roofline figures use. */
+int npbx_stream_copy(void *dst, const void *src, int64_t bytes, void *stream);
+
 /* ---- 2-D 5-point Jacobi (jacobi_2d_numpy.py:4-10) -------------------------
  * A, B: N x N.  Runs TSTEPS-1 ping-pong iterations (2*(TSTEPS-1) sweeps,
  * A->B then B->A), interior only, boundaries never written.

@@ -3,16 +3,22 @@
 // Layout: N x N x N, C order (i slowest, k contiguous).  A sweep writes the
 // interior [1, N-2]^3 of dst from src; boundary faces are never written.
 //
-// Plain sweep (2.5-D): a 256-thread workgroup covers a 64 (k, one wave,
-// coalesced) x 4 (j) tile of columns and each thread marches along i over
-// kRows planes, carrying the i-1 / i / i+1 values of its column in registers.
-// The j+-1 and k+-1 neighbours of plane i come from the same or adjacent
-// cache lines that neighbouring lanes and waves fetch (L1/L2 hits), so each
-// src element crosses HBM about once per sweep.
+// Plain sweep (2.5-D), vector form: each lane owns V consecutive k (one
+// 16-byte load: 4 FP32, 2 FP64, 8 BF16), a wave spans 64*V k, a 256-thread
+// workgroup is one wave-row x 4 j-rows, and each thread marches along i over
+// kPlanes planes carrying the i-1 / i / i+1 vectors in registers.  Loads for
+// kUnroll planes (i+1, j+-1 vectors and the two k-edge scalars) are issued
+// before any of them is used, so every lane keeps several 16-byte HBM
+// requests in flight.  The j+-1 rows and k edges are L1/L2 hits.  Tiles are
+// numbered XCD by XCD (xcd_tile) so neighbours share an L2.  Each src element
+// crosses HBM about once per sweep.
+//
+// Scalar form (rows not 16-byte aligned): one k per lane, same marching.
 //
 // Arithmetic follows the reference: per axis 0.125*((x[+1] - 2*c) + x[-1]),
 // the axis terms summed i, j, k left to right, then + c.
 #include "common.h"
+#include "vecio.h"
 
 namespace npbx {
 namespace {
@@ -57,9 +63,82 @@ __global__ __launch_bounds__(kLanes* kCols) void heat3d_sweep(const S* __restric
   }
 }
 
+constexpr int kPlanes = 16;  // planes per workgroup (i segment)
+constexpr int kUnroll = 4;   // planes whose loads are issued together
+
+template <class S>
+__global__ __launch_bounds__(kLanes* kCols) void heat3d_sweep_vec(const S* __restrict__ src,
+                                                                 S* __restrict__ dst, int n,
+                                                                 int tiles_k, int tiles_j) {
+  using T = Storage<S>;
+  using C = typename T::C;
+  constexpr int V = Vec16<S>::N;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int tk = t % tiles_k;
+  const int tj = (t / tiles_k) % tiles_j;
+  const int ti = t / (tiles_k * tiles_j);
+  const int k0 = (tk * kLanes + threadIdx.x) * V;
+  const int j = 1 + tj * kCols + threadIdx.y;
+  const int i0 = 1 + ti * kPlanes;
+  if (k0 >= n || j > n - 2) return;
+  const int i_end = min(i0 + kPlanes, n - 1);
+  const int64_t plane = static_cast<int64_t>(n) * n;
+  const int64_t col = static_cast<int64_t>(j) * n + k0;
+  const bool has_km = k0 > 0, has_kp = k0 + V < n;
+  const bool full = k0 >= 1 && k0 + V <= n - 1;  // all V points interior in k
+
+  C below[V], centre[V];
+  load16(src + col + (i0 - 1) * plane, below);
+  load16(src + col + i0 * plane, centre);
+  for (int i = i0; i < i_end; i += kUnroll) {
+    C above[kUnroll][V], jp[kUnroll][V], jm[kUnroll][V], km[kUnroll], kp[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      // clamp so the loads of planes past the segment end stay in bounds
+      const int64_t q = col + min(i + u, n - 2) * plane;
+      load16(src + q + plane, above[u]);
+      load16(src + q + n, jp[u]);
+      load16(src + q - n, jm[u]);
+      km[u] = has_km ? T::load(src[q - 1]) : C(0);
+      kp[u] = has_kp ? T::load(src[q + V]) : C(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      if (i + u >= i_end) break;
+      C out[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const C left = v == 0 ? km[u] : centre[v - 1];
+        const C right = v == V - 1 ? kp[u] : centre[v + 1];
+        out[v] = heat_point<C>(centre[v], above[u][v], below[v], jp[u][v], jm[u][v], right, left);
+      }
+      S* o = dst + col + static_cast<int64_t>(i + u) * plane;
+      if (full) {
+        store16(o, out);
+      } else {
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+          if (k0 + v >= 1 && k0 + v <= n - 2) o[v] = T::store(out[v]);
+      }
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        below[v] = centre[v];
+        centre[v] = above[u][v];
+      }
+    }
+  }
+}
+
 template <class S>
 int run_sweeps(S* A, S* B, int n, int64_t first, int64_t nsweeps, hipStream_t stream) {
+  constexpr int V = Vec16<S>::N;
   const dim3 block(kLanes, kCols);
+  const bool vec = n % V == 0 && aligned16(A) && aligned16(B);
+  const int tiles_k = static_cast<int>(cdiv(n, kLanes * V));
+  const int tiles_j = static_cast<int>(cdiv(n - 2, kCols));
+  const int tiles_i = static_cast<int>(cdiv(n - 2, kPlanes));
+  const int64_t nblocks = int64_t(tiles_k) * tiles_j * tiles_i;
+  NPBX_REQUIRE(nblocks < (int64_t(1) << 31), "heat3d: grid too large");
   const dim3 grid(static_cast<unsigned>(cdiv(n - 2, kLanes)),
                   static_cast<unsigned>(cdiv(n - 2, kCols)),
                   static_cast<unsigned>(cdiv(n - 2, kRows)));
@@ -67,8 +146,14 @@ int run_sweeps(S* A, S* B, int n, int64_t first, int64_t nsweeps, hipStream_t st
     const bool even = (s % 2) == 0;
     const S* src = even ? A : B;
     S* dst = even ? B : A;
-    hipLaunchKernelGGL(heat3d_sweep<S>, grid, block, 0, stream, src, dst, n);
-    NPBX_LAUNCHED("heat3d_sweep");
+    if (vec) {
+      hipLaunchKernelGGL(heat3d_sweep_vec<S>, dim3(static_cast<unsigned>(nblocks)), block, 0,
+                         stream, src, dst, n, tiles_k, tiles_j);
+      NPBX_LAUNCHED("heat3d_sweep_vec");
+    } else {
+      hipLaunchKernelGGL(heat3d_sweep<S>, grid, block, 0, stream, src, dst, n);
+      NPBX_LAUNCHED("heat3d_sweep");
+    }
   }
   return NPBX_OK;
 }
