@@ -95,7 +95,9 @@ int npbx_stream_copy(void *dst, const void *src, int64_t bytes, void *stream);
 /* ---- 2-D 5-point Jacobi (jacobi_2d_numpy.py:4-10) -------------------------
  * A, B: N x N.  Runs TSTEPS-1 ping-pong iterations (2*(TSTEPS-1) sweeps,
  * A->B then B->A), interior only, boundaries never written.
- * tblock: sweeps fused per HBM round trip (0 or 1 = one launch per sweep).
+ * tblock: sweeps fused per HBM round trip (0 or 1 = one launch per sweep;
+ * 2..4 FP32, 2 FP64/BF16 = temporal-blocked tiles, bitwise the same result;
+ * needs N a multiple of the elements per 16 bytes).
  */
 int npbx_jacobi2d(int dtype, void *A, void *B, int64_t N, int64_t tsteps,
                   int tblock, void *stream);
@@ -110,6 +112,67 @@ int npbx_heat3d(int dtype, void *A, void *B, int64_t N, int64_t tsteps,
                 int tblock, void *stream);
 int npbx_heat3d_sweeps(int dtype, void *A, void *B, int64_t N, int64_t first,
                        int64_t nsweeps, int tblock, void *stream);
+
+/* ---- batched grids: many independent grids in persistent launches --------
+ * No reference counterpart: the batched and serving forms of the two
+ * explicit kernels.  A plan fixes a list of grids (each with its own side,
+ * arrays and sweep range, all of one dtype) and a temporal-blocking depth
+ * `tblock` (2..4 FP32, 2 FP64/BF16); every launch of the plan runs one round
+ * of up to `tblock` sweeps for every grid that still has one, as one
+ * persistent kernel over a device work-tile queue.  Results are bitwise
+ * those of the per-grid entry points.  `seg` = heat planes per tile (0 =
+ * default).  Each grid's side must be a multiple of 16 bytes of elements.
+ * npbx_plan_capture records the plan's launches into a HIP graph on a
+ * non-default stream; later npbx_plan_launch calls replay the graph. */
+typedef enum npbx_kind {
+  NPBX_KIND_JACOBI2D = 0,
+  NPBX_KIND_HEAT3D = 1
+} npbx_kind;
+
+typedef struct npbx_grid_spec {
+  int32_t kind;    /* npbx_kind */
+  int32_t dtype;   /* npbx_dtype, the same for every grid of a plan */
+  int64_t n;       /* side length */
+  void *A;         /* the reference's A and B arrays (device) */
+  void *B;
+  int64_t first;   /* first sweep index (its parity picks the source array) */
+  int64_t nsweeps; /* sweeps to run */
+} npbx_grid_spec;
+
+int npbx_plan_create(const npbx_grid_spec *grids, int32_t count,
+                     int32_t tblock, int32_t seg, void **plan);
+int npbx_plan_launch(void *plan, void *stream);
+int npbx_plan_capture(void *plan, void *stream);
+int npbx_plan_info(void *plan, int64_t *launches, int64_t *tiles,
+                   int64_t *lups);
+int npbx_plan_destroy(void *plan);
+
+/* Serving form: one launch advances each listed grid by one round of its own
+ * schedule (continuous batching: the caller changes the list between
+ * launches).  Round i reads level `level0` of grid i from X (A or B) and
+ * writes level level0+depth to the other array; depth <= tblock.  Rounds
+ * that follow npbx_round_depths() for a grid's sweep range leave A and B
+ * exactly as npbx_jacobi2d_sweeps / npbx_heat3d_sweeps would.  `queue` is a
+ * work-queue state from npbx_queue_create, used by one stream at a time. */
+typedef struct npbx_round_spec {
+  int32_t kind;
+  int32_t dtype;
+  int64_t n;
+  void *A;
+  void *B;
+  const void *X;
+  void *Y;
+  int64_t level0;
+  int32_t depth;
+  int32_t reserved;
+} npbx_round_spec;
+
+int npbx_round_depths(int64_t nsweeps, int32_t tblock, int32_t *depths,
+                      int32_t capacity, int32_t *count);
+int npbx_queue_create(void **queue);
+int npbx_queue_destroy(void *queue);
+int npbx_rounds_launch(const npbx_round_spec *rounds, int32_t count,
+                       int32_t tblock, int32_t seg, void *queue, void *stream);
 
 /* ---- horizontal diffusion (hdiff_numpy.py:5-29) ----------------------------
  * in_field: (I+4, J+4, K); out_field, coeff: (I, J, K).  nsteps == 1 is the

@@ -24,6 +24,25 @@ BF16 = 2
 _STATUS = {EINVAL: "EINVAL", EHIP: "EHIP", ETIMEDOUT: "ETIMEDOUT", ENOMEM: "ENOMEM",
            EUNSUPPORTED: "EUNSUPPORTED"}
 
+KIND_JACOBI2D = 0
+KIND_HEAT3D = 1
+
+
+class GridSpec(ctypes.Structure):
+    """npbx_grid_spec of include/npbx.h."""
+    _fields_ = [("kind", ctypes.c_int32), ("dtype", ctypes.c_int32), ("n", ctypes.c_int64),
+                ("A", ctypes.c_void_p), ("B", ctypes.c_void_p), ("first", ctypes.c_int64),
+                ("nsweeps", ctypes.c_int64)]
+
+
+class RoundSpec(ctypes.Structure):
+    """npbx_round_spec of include/npbx.h."""
+    _fields_ = [("kind", ctypes.c_int32), ("dtype", ctypes.c_int32), ("n", ctypes.c_int64),
+                ("A", ctypes.c_void_p), ("B", ctypes.c_void_p), ("X", ctypes.c_void_p),
+                ("Y", ctypes.c_void_p), ("level0", ctypes.c_int64), ("depth", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
 _int = ctypes.c_int
@@ -52,6 +71,18 @@ SIGNATURES = {
     "npbx_jacobi2d_sweeps": [_int, _vp, _vp, _i64, _i64, _i64, _int, _vp],
     "npbx_heat3d": [_int, _vp, _vp, _i64, _i64, _int, _vp],
     "npbx_heat3d_sweeps": [_int, _vp, _vp, _i64, _i64, _i64, _int, _vp],
+    "npbx_plan_create": [ctypes.POINTER(GridSpec), ctypes.c_int32, ctypes.c_int32,
+                         ctypes.c_int32, ctypes.POINTER(_vp)],
+    "npbx_plan_launch": [_vp, _vp],
+    "npbx_plan_capture": [_vp, _vp],
+    "npbx_plan_info": [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)],
+    "npbx_plan_destroy": [_vp],
+    "npbx_round_depths": [_i64, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32,
+                          ctypes.POINTER(ctypes.c_int32)],
+    "npbx_queue_create": [ctypes.POINTER(_vp)],
+    "npbx_queue_destroy": [_vp],
+    "npbx_rounds_launch": [ctypes.POINTER(RoundSpec), ctypes.c_int32, ctypes.c_int32,
+                           ctypes.c_int32, _vp, _vp],
     "npbx_hdiff": [_int, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _vp],
     "npbx_vadv": [_int, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _i64, _i64, _i64, _vp],
 }

@@ -112,6 +112,11 @@ int dispatch_dtype(int dtype, F&& f) {
   }
 }
 
+// Temporal-blocked sweeps of one grid (kind 0 = Jacobi 2-D, 1 = heat 3-D)
+// through the tile kernel: depth T (>= 2) per HBM round trip.
+int tiled_sweeps(int kind, int dtype, void* A, void* B, int n, int64_t first, int64_t nsweeps,
+                 int T, int seg, hipStream_t stream);
+
 // Per-device scratch owned by the library (grow-only, freed at exit).  Used
 // by kernels that need storage beyond their inputs and outputs.
 int workspace(size_t bytes, void** ptr);

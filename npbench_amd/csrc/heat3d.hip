@@ -167,6 +167,8 @@ int heat_entry(int dtype, void* A, void* B, int64_t N, int64_t first, int64_t ns
   NPBX_REQUIRE(A && B, "heat3d: null array");
   NPBX_REQUIRE(A != B, "heat3d: A and B must be distinct arrays");
   hipStream_t s = as_stream(stream);
+  if (tblock >= 2)
+    return tiled_sweeps(1, dtype, A, B, static_cast<int>(N), first, nsweeps, tblock, 0, s);
   return dispatch_dtype(dtype, [&](auto tag) {
     using S = decltype(tag);
     return run_sweeps<S>(static_cast<S*>(A), static_cast<S*>(B), static_cast<int>(N), first,

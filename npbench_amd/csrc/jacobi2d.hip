@@ -152,6 +152,8 @@ int jacobi_entry(int dtype, void* A, void* B, int64_t N, int64_t first, int64_t 
   NPBX_REQUIRE(A != B, "jacobi2d: A and B must be distinct arrays");
   NPBX_REQUIRE(N * N <= (int64_t(1) << 31), "jacobi2d: N*N exceeds 2^31 elements");
   hipStream_t s = as_stream(stream);
+  if (tblock >= 2)
+    return tiled_sweeps(0, dtype, A, B, static_cast<int>(N), first, nsweeps, tblock, 0, s);
   return dispatch_dtype(dtype, [&](auto tag) {
     using S = decltype(tag);
     return run_sweeps<S>(static_cast<S*>(A), static_cast<S*>(B), static_cast<int>(N), first,

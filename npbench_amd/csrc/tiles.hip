@@ -1,0 +1,824 @@
+// Temporal-blocked tile kernels for the explicit stencils, and the work-tile
+// queue that runs many grids of uneven size in one persistent launch.
+//
+// A tile advances a block of one grid by `depth` (<= T) sweeps in one HBM
+// round trip: it reads level L0 once (tile plus a halo of T cells), computes
+// the intermediate levels on chip, and writes only level L0+depth of its
+// central region.  Arithmetic per point is exactly the plain sweep's
+// (jacobi_2d_numpy.py:7-10, heat_3d_numpy.py:7-20 order), and every level
+// kept on chip is rounded through the storage type as a plain sweep's store
+// would round it, so results are bitwise independent of T and of the tiling.
+//
+// Workgroup: 256 threads = 4 waves; lane l owns V consecutive points along
+// the contiguous axis (one 16-byte vector: 4 FP32, 2 FP64, 8 BF16), so a
+// wave spans KT = 64*V points of a row and the row's left/right neighbours
+// cross lanes by shuffle.  Wave w owns R consecutive rows; rows r-1 / r+1
+// are in the lane's own registers except at a wave's first and last row,
+// which are exchanged through LDS (two rows per wave per level).
+//
+// heat (3-D): the tile is 4R rows (j) x KT (k) and marches along i through
+// an output segment of `seg` planes plus T planes of halo on each side.
+// Level l keeps a three-plane register window (i-1, i, i+1); each step loads
+// one new level-0 plane and pushes it through the T levels (2.5-D spatial
+// plus temporal blocking).  Only level-0 planes are read from HBM and only
+// level-T planes of the output segment are written.
+//
+// jacobi (2-D): the tile is 4R rows x KT columns held in registers; the T
+// levels are computed in place with an LDS edge-row exchange per level.
+//
+// Fixed points: points on the global boundary keep the boundary value of the
+// level's parity (A for even levels, B for odd), loaded from A or B.
+//
+// Queue: a launch carries up to 48 grid rounds by value (RoundBatch); a
+// persistent grid of workgroups pulls tile indices from an atomic counter and
+// the last workgroup to find the queue empty resets it, so consecutive
+// launches and graph replays need no memset and no device-side tile list.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+#include "tiles.h"
+#include "vecio.h"
+
+namespace npbx {
+namespace {
+
+constexpr int kWaves = 4;
+constexpr int kThreads = 64 * kWaves;
+constexpr int kRowsHeat = 4;  // rows per wave, heat
+
+// Rows per wave of a Jacobi tile (R*V <= 64 so a lane's points fit one
+// 64-bit fixed-point mask).
+constexpr int jacobi_rows(int v) { return v >= 8 ? 8 : 8; }
+
+// Largest depth whose register windows fit: FP64 is also limited to the one
+// vector (2 points) of halo along the contiguous axis.
+template <class S>
+struct MaxDepth {
+  static constexpr int value = sizeof(S) == 4 ? 4 : 2;
+};
+
+template <class C>
+__device__ __forceinline__ C heat_point(C c, C ip, C im, C jp, C jm, C kp, C km) {
+  const C c2 = static_cast<C>(2.0) * c;
+  const C q = static_cast<C>(0.125);
+  const C ti = q * ((ip - c2) + im);
+  const C tj = q * ((jp - c2) + jm);
+  const C tk = q * ((kp - c2) + km);
+  return ((ti + tj) + tk) + c;
+}
+
+template <class C>
+__device__ __forceinline__ C jacobi_point(C c, C w, C e, C s, C n) {
+  C acc = c + w;
+  acc = acc + e;
+  acc = acc + s;
+  acc = acc + n;
+  return static_cast<C>(0.2) * acc;
+}
+
+// Boundary source of a level: A for even levels, B for odd.  Selecting
+// between two pointer values (not indexing the descriptor) keeps the
+// descriptor in registers.
+template <class S>
+__device__ __forceinline__ const S* bnd(const S* a, const S* b, int level_parity) {
+  return (level_parity & 1) ? b : a;
+}
+
+// Contiguous-axis tile start and output range for a side of length n.
+template <int KT, int V>
+__device__ __forceinline__ void col_range(int n, int tiles_k, int tk, int& kl, int& klo, int& khi) {
+  if (tiles_k == 1 && n <= KT) {
+    kl = 0;
+    klo = 1;
+    khi = n - 1;
+  } else {
+    const int wk = KT - 2 * V;
+    kl = tk * wk - V;
+    klo = max(kl + V, 1);
+    khi = min(kl + KT - V, n - 1);
+  }
+}
+
+// ---------------------------------------------------------------- heat 3-D
+template <class S, int T>
+__device__ __forceinline__ void heat_tile(const GridRound g, int tile,
+                                          typename Storage<S>::C* lds) {
+  using St = Storage<S>;
+  using C = typename St::C;
+  constexpr int V = Vec16<S>::N;
+  constexpr int R = kRowsHeat;
+  constexpr int KT = 64 * V;
+  constexpr int JT = kWaves * R;
+  const int n = g.n, d = g.depth;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int tk = tile % g.tiles_k;
+  const int rest = tile / g.tiles_k;
+  const int tj = rest % g.tiles_j;
+  const int ti = rest / g.tiles_j;
+  const int wj = JT - 2 * T;
+  const int jl = 1 + tj * wj - T;
+  int kl, klo, khi;
+  col_range<KT, V>(n, g.tiles_k, tk, kl, klo, khi);
+  const int jlo = max(jl + T, 1), jhi = min(jl + T + wj, n - 1);
+  const int i0 = 1 + ti * g.seg, i1 = min(i0 + g.seg, n - 1);
+  const int k0 = kl + lane * V;
+  const bool vlive = k0 >= 0 && k0 + V <= n;
+  const int64_t plane = static_cast<int64_t>(n) * n;
+  const S* X = static_cast<const S*>(g.X);
+  S* Y = static_cast<S*>(g.Y);
+  const S* gA = static_cast<const S*>(g.A);
+  const S* gB = static_cast<const S*>(g.B);
+
+  bool row_live[R];
+  uint32_t fixed = 0;  // bit r*V+v: global boundary in j or k
+  int64_t rowoff[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int j = jl + w * R + r;
+    row_live[r] = vlive && j >= 0 && j < n;
+    rowoff[r] = static_cast<int64_t>(j) * n + k0;
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const int k = k0 + v;
+      if (row_live[r] && (j == 0 || j == n - 1 || k == 0 || k == n - 1)) fixed |= 1u << (r * V + v);
+    }
+  }
+
+  // Whole-plane read of level parity p at plane q: boundary planes come
+  // from A/B; interior planes from `src` with j/k boundary points patched.
+  auto read_plane = [&](C (&dst)[R][V], int q, int p, const S* src) {
+    const S* b = bnd<S>(gA, gB, p);
+    const bool bplane = q == 0 || q == n - 1;
+    const S* from = bplane ? b : src;
+    const int64_t base = static_cast<int64_t>(q) * plane;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (row_live[r]) load16(from + base + rowoff[r], dst[r]);
+    if (fixed && !bplane) {
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+          if (fixed & (1u << (r * V + v))) dst[r][v] = St::load(b[base + rowoff[r] + v]);
+    }
+  };
+
+  C win[T][3][R][V];  // per level: [0] plane q-1, [1] plane q, [2] plane q+1
+  const int pbeg = max(i0 - d, 0), pend = i1 - 1 + d;
+  for (int P = pbeg; P <= pend; ++P) {
+    if (P <= n - 1) read_plane(win[0][2], P, g.parity0, X);
+#pragma unroll
+    for (int l = 0; l < T; ++l) {
+      if (l >= d) break;
+      const int q = P - 1 - l;  // centre plane of level l; level l+1 is computed there
+      C* L = lds + l * (kWaves * 2 * KT);
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        L[(w * 2 + 0) * KT + lane * V + v] = win[l][1][0][v];
+        L[(w * 2 + 1) * KT + lane * V + v] = win[l][1][R - 1][v];
+      }
+      __syncthreads();
+      C nw[R][V];
+      if (q >= 1 && q <= n - 2) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          C jm[V], jp[V];
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            jm[v] = r > 0 ? win[l][1][r - 1][v]
+                          : (w > 0 ? L[((w - 1) * 2 + 1) * KT + lane * V + v] : win[l][1][r][v]);
+            jp[v] = r < R - 1 ? win[l][1][r + 1][v]
+                              : (w < kWaves - 1 ? L[((w + 1) * 2) * KT + lane * V + v]
+                                                : win[l][1][r][v]);
+          }
+          const C left = __shfl_up(win[l][1][r][V - 1], 1, 64);
+          const C right = __shfl_down(win[l][1][r][0], 1, 64);
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            const C km = v > 0 ? win[l][1][r][v - 1] : left;
+            const C kp = v < V - 1 ? win[l][1][r][v + 1] : right;
+            // Round through the storage type as the plain sweep's store does
+            // (identity for FP32/FP64), so results do not depend on T.
+            nw[r][v] = St::load(St::store(heat_point<C>(win[l][1][r][v], win[l][2][r][v],
+                                                        win[l][0][r][v], jp[v], jm[v], kp, km)));
+          }
+        }
+        if (fixed) {
+          const S* b = bnd<S>(gA, gB, g.parity0 + l + 1);
+          const int64_t base = static_cast<int64_t>(q) * plane;
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int v = 0; v < V; ++v)
+              if (fixed & (1u << (r * V + v))) nw[r][v] = St::load(b[base + rowoff[r] + v]);
+        }
+      } else if (q == 0 || q == n - 1) {
+        read_plane(nw, q, g.parity0 + l + 1, X);
+      }
+      // slide level l's window up one plane
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          win[l][0][r][v] = win[l][1][r][v];
+          win[l][1][r][v] = win[l][2][r][v];
+        }
+      if (l + 1 < d) {
+        // l + 1 < d <= T; the clamp only keeps the (unrolled) index in range
+        constexpr int kLast = T - 1;
+        const int nxt = l + 1 < kLast ? l + 1 : kLast;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+          for (int v = 0; v < V; ++v) win[nxt][2][r][v] = nw[r][v];
+      } else if (q >= i0 && q < i1) {
+        const int64_t base = static_cast<int64_t>(q) * plane;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int j = jl + w * R + r;
+          if (!row_live[r] || j < jlo || j >= jhi) continue;
+          S* o = Y + base + rowoff[r];
+          if (k0 >= klo && k0 + V <= khi) {
+            store16(o, nw[r]);
+          } else {
+#pragma unroll
+            for (int v = 0; v < V; ++v)
+              if (k0 + v >= klo && k0 + v < khi) o[v] = St::store(nw[r][v]);
+          }
+        }
+      }
+    }
+    if (d == 1) __syncthreads();  // LDS of level 0 is rewritten next step
+  }
+}
+
+// ---------------------------------------------------------------- jacobi 2-D
+template <class S, int T>
+__device__ __forceinline__ void jacobi_tile(const GridRound g, int tile,
+                                            typename Storage<S>::C* lds) {
+  using St = Storage<S>;
+  using C = typename St::C;
+  constexpr int V = Vec16<S>::N;
+  constexpr int R = jacobi_rows(V);
+  constexpr int KT = 64 * V;
+  constexpr int JT = kWaves * R;
+  const int n = g.n, d = g.depth;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int tk = tile % g.tiles_k;
+  const int tj = tile / g.tiles_k;
+  const int wj = JT - 2 * T;
+  const int il = 1 + tj * wj - T;  // first row of the tile
+  int kl, klo, khi;
+  col_range<KT, V>(n, g.tiles_k, tk, kl, klo, khi);
+  const int ilo = max(il + T, 1), ihi = min(il + T + wj, n - 1);
+  const int k0 = kl + lane * V;
+  const bool vlive = k0 >= 0 && k0 + V <= n;
+  const S* X = static_cast<const S*>(g.X);
+  S* Y = static_cast<S*>(g.Y);
+  const S* gA = static_cast<const S*>(g.A);
+  const S* gB = static_cast<const S*>(g.B);
+
+  bool row_live[R];
+  uint64_t fixed = 0;  // bit r*V+v (R*V <= 64)
+  uint64_t brow = 0;   // rows that are boundary rows
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = il + w * R + r;
+    row_live[r] = vlive && i >= 0 && i < n;
+    if (row_live[r] && (i == 0 || i == n - 1)) brow |= 1ull << r;
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const int k = k0 + v;
+      if (row_live[r] && (i == 0 || i == n - 1 || k == 0 || k == n - 1))
+        fixed |= 1ull << (r * V + v);
+    }
+  }
+  auto off = [&](int r) { return static_cast<int64_t>(il + w * R + r) * n + k0; };
+
+  C cur[R][V];
+  {
+    const S* b = bnd<S>(gA, gB, g.parity0);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (row_live[r]) load16(((brow >> r) & 1) ? b + off(r) : X + off(r), cur[r]);
+    if (fixed) {
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+          if ((fixed >> (r * V + v)) & 1) cur[r][v] = St::load(b[off(r) + v]);
+    }
+  }
+  for (int l = 0; l < d; ++l) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      lds[(w * 2 + 0) * KT + lane * V + v] = cur[0][v];
+      lds[(w * 2 + 1) * KT + lane * V + v] = cur[R - 1][v];
+    }
+    __syncthreads();
+    // Rows are updated in place top to bottom: row r's north neighbour is
+    // the saved old row r-1, its south neighbour cur[r+1] is still old.
+    const S* b = bnd<S>(gA, gB, g.parity0 + l + 1);
+    C prev[V];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      C north[V], south[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        north[v] = r > 0 ? prev[v]
+                         : (w > 0 ? lds[((w - 1) * 2 + 1) * KT + lane * V + v] : cur[r][v]);
+        south[v] = r < R - 1 ? cur[r + 1][v]
+                             : (w < kWaves - 1 ? lds[((w + 1) * 2) * KT + lane * V + v] : cur[r][v]);
+      }
+      const C left = __shfl_up(cur[r][V - 1], 1, 64);
+      const C right = __shfl_down(cur[r][0], 1, 64);
+      C nw[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const C west = v > 0 ? cur[r][v - 1] : left;
+        const C east = v < V - 1 ? cur[r][v + 1] : right;
+        // rounded through the storage type, as the plain sweep's store does
+        nw[v] = St::load(St::store(jacobi_point<C>(cur[r][v], west, east, south[v], north[v])));
+        if ((fixed >> (r * V + v)) & 1) nw[v] = St::load(b[off(r) + v]);
+      }
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        prev[v] = cur[r][v];
+        cur[r][v] = nw[v];
+      }
+    }
+    __syncthreads();  // every wave has read the edge rows before they change
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = il + w * R + r;
+    if (!row_live[r] || i < ilo || i >= ihi) continue;
+    S* o = Y + off(r);
+    if (k0 >= klo && k0 + V <= khi) {
+      store16(o, cur[r]);
+    } else {
+#pragma unroll
+      for (int v = 0; v < V; ++v)
+        if (k0 + v >= klo && k0 + v < khi) o[v] = St::store(cur[r][v]);
+    }
+  }
+}
+
+template <class S, int T>
+struct TileLds {
+  using C = typename Storage<S>::C;
+  static constexpr int V = Vec16<S>::N;
+  static constexpr int KT = 64 * V;
+  static constexpr int HEAT = T * kWaves * 2 * KT;
+  static constexpr int JACOBI = kWaves * 2 * KT;  // one exchange buffer
+  static constexpr int ELEMS = HEAT > JACOBI ? HEAT : JACOBI;
+};
+
+// Tile t of a launch belongs to the round g with cum[g] <= t < cum[g+1].
+__device__ __forceinline__ int round_of(const RoundBatch& b, int t) {
+  int lo = 0, hi = b.count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (b.cum[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <class S, int T>
+__device__ __forceinline__ void run_tile(const RoundBatch& b, int t,
+                                         typename Storage<S>::C* lds) {
+  const int g = round_of(b, t);
+  const GridRound r = b.round[g];
+  if (r.kind == kHeat3D)
+    heat_tile<S, T>(r, t - b.cum[g], lds);
+  else
+    jacobi_tile<S, T>(r, t - b.cum[g], lds);
+}
+
+// queue == nullptr: one workgroup per tile (tile = blockIdx.x).  Otherwise a
+// persistent workgroup pulls tile indices from `queue` until they run out.
+template <class S, int T>
+__global__ __launch_bounds__(kThreads) void stencil_tiles(const RoundBatch batch,
+                                                          QueueState* queue) {
+  using C = typename Storage<S>::C;
+  __shared__ C lds[TileLds<S, T>::ELEMS];
+  const int ntiles = batch.cum[batch.count];
+  if (queue == nullptr) {
+    run_tile<S, T>(batch, blockIdx.x, lds);
+    return;
+  }
+  __shared__ int s_tile;
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(&queue->next, 1);
+    __syncthreads();
+    const int t = s_tile;
+    __syncthreads();
+    if (t >= ntiles) break;
+    run_tile<S, T>(batch, t, lds);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&queue->done, 1) == static_cast<int>(gridDim.x) - 1) {
+      atomicExch(&queue->next, 0);
+      atomicExch(&queue->done, 0);
+    }
+  }
+}
+
+// ------------------------------------------------------------- host planning
+int max_depth(int dtype) {
+  switch (dtype) {
+    case NPBX_F64: return MaxDepth<double>::value;
+    case NPBX_F32: return MaxDepth<float>::value;
+    case NPBX_BF16: return MaxDepth<bf16>::value;
+    default: return 0;
+  }
+}
+
+int vec_elems(int dtype) { return static_cast<int>(16 / dtype_size(dtype)); }
+
+// Fills the tiling fields of a GridRound for side n and max depth T.
+void tile_grid(GridRound& g, int kind, int n, int dtype, int T, int seg) {
+  const int V = vec_elems(dtype);
+  const int KT = 64 * V;
+  g.tiles_k = n <= KT ? 1 : static_cast<int>(cdiv(n, KT - 2 * V));
+  if (kind == kHeat3D) {
+    g.tiles_j = static_cast<int>(cdiv(n - 2, kWaves * kRowsHeat - 2 * T));
+    g.seg = seg;
+  } else {
+    g.tiles_j = static_cast<int>(cdiv(n - 2, kWaves * jacobi_rows(V) - 2 * T));
+    g.seg = 0;
+  }
+}
+
+int64_t tiles_per_round(const GridRound& g) {
+  int64_t t = int64_t(g.tiles_k) * g.tiles_j;
+  if (g.kind == kHeat3D) t *= cdiv(g.n - 2, g.seg);
+  return t;
+}
+
+// Depths of the rounds that advance nsweeps sweeps so that, as in the
+// reference, level first+nsweeps ends in its home array and level
+// first+nsweeps-1 in the other: rounds of depth <= T cover nsweeps-1 sweeps,
+// their count has the parity of nsweeps-1 (each round flips the array the
+// state lives in), and a final depth-1 round performs the last sweep.
+std::vector<int> round_depths(int64_t nsweeps, int T) {
+  std::vector<int> out;
+  if (nsweeps <= 0) return out;
+  const int64_t m = nsweeps - 1;
+  int64_t R = cdiv(m, T);
+  if ((R & 1) != (m & 1)) ++R;
+  for (int64_t r = 0; r < R; ++r) out.push_back(static_cast<int>(m / R + (r < m % R ? 1 : 0)));
+  out.push_back(1);
+  return out;
+}
+
+template <class S, int T>
+int launch_tiles(const RoundBatch& b, QueueState* queue, int nblocks, hipStream_t stream) {
+  hipLaunchKernelGGL((stencil_tiles<S, T>), dim3(static_cast<unsigned>(nblocks)), dim3(kThreads),
+                     0, stream, b, queue);
+  NPBX_LAUNCHED("stencil_tiles");
+  return NPBX_OK;
+}
+
+template <class S>
+int dispatch_T(int T, const RoundBatch& b, QueueState* queue, int nblocks, hipStream_t stream) {
+  switch (T) {
+    case 2: return launch_tiles<S, 2>(b, queue, nblocks, stream);
+    case 3:
+      if constexpr (MaxDepth<S>::value >= 3) return launch_tiles<S, 3>(b, queue, nblocks, stream);
+      break;
+    case 4:
+      if constexpr (MaxDepth<S>::value >= 4) return launch_tiles<S, 4>(b, queue, nblocks, stream);
+      break;
+    default: break;
+  }
+  return fail(NPBX_EUNSUPPORTED, "tile depth T=%d not supported for this dtype", T);
+}
+
+int launch_any(int dtype, int T, const RoundBatch& b, QueueState* queue, int nblocks,
+               hipStream_t stream) {
+  return dispatch_dtype(dtype, [&](auto tag) {
+    using S = decltype(tag);
+    return dispatch_T<S>(T, b, queue, nblocks, stream);
+  });
+}
+
+// Workgroups that fit on the device at once for the persistent grid.
+int resident_blocks(int dtype, int T) {
+  int dev = 0, cus = 0, per_cu = 1;
+  if (hipGetDevice(&dev) != hipSuccess) return 1024;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    cus = 256;
+  auto occ = [&](auto kern) {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kern, kThreads, 0) == hipSuccess && b > 0)
+      per_cu = b;
+  };
+  dispatch_dtype(dtype, [&](auto tag) {
+    using S = decltype(tag);
+    if (T == 2) occ(stencil_tiles<S, 2>);
+    if constexpr (MaxDepth<S>::value >= 4) {
+      if (T == 3) occ(stencil_tiles<S, 3>);
+      if (T == 4) occ(stencil_tiles<S, 4>);
+    }
+    return NPBX_OK;
+  });
+  return cus * per_cu;
+}
+
+}  // namespace
+
+// Single grid, temporal-blocked: rounds of depth <= T through the tile
+// kernel (one workgroup per tile), then the final depth-1 round.
+int tiled_sweeps(int kind, int dtype, void* A, void* B, int n, int64_t first, int64_t nsweeps,
+                 int T, int seg, hipStream_t stream) {
+  const int tmax = max_depth(dtype);
+  NPBX_REQUIRE(T >= 2 && T <= tmax, "tblock=%d outside [2, %d] for %s", T, tmax,
+               dtype_name(dtype));
+  NPBX_REQUIRE(n % vec_elems(dtype) == 0 && aligned16(A) && aligned16(B),
+               "temporal blocking needs rows of whole 16-byte vectors");
+  GridRound g{};
+  g.A = A;
+  g.B = B;
+  g.n = n;
+  g.kind = kind;
+  tile_grid(g, kind, n, dtype, T, seg > 0 ? seg : 32);
+  const int64_t ntiles = tiles_per_round(g);
+  NPBX_REQUIRE(ntiles < (int64_t(1) << 31), "too many tiles");
+  int64_t level = first;
+  // Round 1 reads the array that holds level `first` in the reference; each
+  // round then writes the other array and the next round reads it.
+  void* X = (first & 1) ? B : A;
+  RoundBatch b{};
+  b.count = 1;
+  b.cum[0] = 0;
+  b.cum[1] = static_cast<int32_t>(ntiles);
+  for (int depth : round_depths(nsweeps, T)) {
+    g.X = X;
+    g.Y = X == A ? B : A;
+    g.parity0 = static_cast<int>(level & 1);
+    g.depth = depth;
+    b.round[0] = g;
+    NPBX_TRY(launch_any(dtype, T, b, nullptr, static_cast<int>(ntiles), stream));
+    X = g.Y;
+    level += depth;
+  }
+  return NPBX_OK;
+}
+
+}  // namespace npbx
+
+// ------------------------------------------------------------------- C ABI
+using namespace npbx;
+
+namespace {
+
+// Validates one grid of a batch and fills the fixed part of its GridRound.
+int check_grid(const char* who, int i, int32_t kind, int32_t dtype, int64_t n, void* A, void* B,
+               int want_dtype, GridRound& g) {
+  NPBX_REQUIRE(dtype == want_dtype, "%s: grid %d dtype differs (one dtype per call)", who, i);
+  NPBX_REQUIRE(kind == NPBX_KIND_JACOBI2D || kind == NPBX_KIND_HEAT3D,
+               "%s: grid %d has unknown kind %d", who, i, kind);
+  const int V = vec_elems(dtype);
+  NPBX_REQUIRE(n >= 3 && n % V == 0,
+               "%s: grid %d side %lld must be >= 3 and a multiple of %d", who, i,
+               static_cast<long long>(n), V);
+  NPBX_REQUIRE(n <= (kind == NPBX_KIND_HEAT3D ? 2048 : 46340), "%s: grid %d too large", who, i);
+  NPBX_REQUIRE(A && B && A != B && aligned16(A) && aligned16(B),
+               "%s: grid %d needs distinct 16-byte aligned A and B", who, i);
+  g = GridRound{};
+  g.A = A;
+  g.B = B;
+  g.n = static_cast<int32_t>(n);
+  g.kind = kind == NPBX_KIND_HEAT3D ? kHeat3D : kJacobi2D;
+  return NPBX_OK;
+}
+
+// Packs grid rounds into launches of at most kMaxBatch rounds.
+struct Packer {
+  std::vector<RoundBatch> out;
+  int64_t tiles = 0;
+  void add(const GridRound& g) {
+    if (out.empty() || out.back().count == kMaxBatch) {
+      out.emplace_back();
+      RoundBatch& b = out.back();
+      std::memset(&b, 0, sizeof(b));
+    }
+    RoundBatch& b = out.back();
+    const int64_t nt = tiles_per_round(g);
+    b.round[b.count] = g;
+    b.cum[b.count + 1] = b.cum[b.count] + static_cast<int32_t>(nt);
+    ++b.count;
+    tiles += nt;
+  }
+};
+
+int64_t grid_lups(const GridRound& g, int64_t sweeps) {
+  const int64_t inner = g.n - 2;
+  return sweeps * (g.kind == kHeat3D ? inner * inner * inner : inner * inner);
+}
+
+struct Plan {
+  int dtype = 0;
+  int T = 0;
+  int nblocks = 0;
+  std::vector<RoundBatch> launches;
+  QueueState* d_queue = nullptr;
+  int64_t ntiles = 0, lups = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+int plan_free(Plan* p) {
+  if (!p) return NPBX_OK;
+  // Release paths ignore errors: the plan is gone either way.
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->d_queue) (void)hipFree(p->d_queue);
+  delete p;
+  return NPBX_OK;
+}
+
+int plan_enqueue(Plan* p, hipStream_t s) {
+  for (const RoundBatch& b : p->launches) {
+    const int nt = b.cum[b.count];
+    if (nt == 0) continue;
+    NPBX_TRY(launch_any(p->dtype, p->T, b, p->d_queue, std::min(nt, p->nblocks), s));
+  }
+  return NPBX_OK;
+}
+
+int new_queue(QueueState** q) {
+  NPBX_HIP(hipMalloc(reinterpret_cast<void**>(q), sizeof(QueueState)));
+  NPBX_HIP(hipMemset(*q, 0, sizeof(QueueState)));
+  return NPBX_OK;
+}
+
+}  // namespace
+
+extern "C" int npbx_plan_create(const npbx_grid_spec* specs, int32_t count, int32_t tblock,
+                                int32_t seg, void** plan) {
+  NPBX_REQUIRE(plan, "plan_create: null output pointer");
+  *plan = nullptr;
+  NPBX_REQUIRE(specs && count > 0, "plan_create: no grids");
+  const int dtype = specs[0].dtype;
+  NPBX_REQUIRE(dtype_size(dtype) != 0, "plan_create: bad dtype %d", dtype);
+  const int tmax = max_depth(dtype);
+  NPBX_REQUIRE(tblock >= 2 && tblock <= tmax, "plan_create: tblock=%d outside [2, %d] for %s",
+               tblock, tmax, dtype_name(dtype));
+  if (seg <= 0) seg = 32;
+  std::vector<GridRound> base(count);
+  std::vector<std::vector<int>> depths(count);
+  size_t nrounds = 0;
+  int64_t lups = 0;
+  for (int i = 0; i < count; ++i) {
+    const npbx_grid_spec& s = specs[i];
+    NPBX_TRY(check_grid("plan_create", i, s.kind, s.dtype, s.n, s.A, s.B, dtype, base[i]));
+    NPBX_REQUIRE(s.first >= 0 && s.nsweeps >= 0, "plan_create: grid %d negative sweeps", i);
+    tile_grid(base[i], base[i].kind, base[i].n, dtype, tblock, seg);
+    depths[i] = round_depths(s.nsweeps, tblock);
+    nrounds = std::max(nrounds, depths[i].size());
+    lups += grid_lups(base[i], s.nsweeps);
+  }
+  // Launch group r runs round r of every grid that has one.
+  auto p = new Plan();
+  p->dtype = dtype;
+  p->T = tblock;
+  p->lups = lups;
+  std::vector<int64_t> level(count);
+  std::vector<void*> X(count);
+  for (int i = 0; i < count; ++i) {
+    level[i] = specs[i].first;
+    X[i] = (level[i] & 1) ? specs[i].B : specs[i].A;
+  }
+  for (size_t r = 0; r < nrounds; ++r) {
+    Packer pk;
+    for (int i = 0; i < count; ++i) {
+      if (r >= depths[i].size()) continue;
+      GridRound g = base[i];
+      g.X = X[i];
+      g.Y = X[i] == specs[i].A ? specs[i].B : specs[i].A;
+      g.parity0 = static_cast<int32_t>(level[i] & 1);
+      g.depth = depths[i][r];
+      pk.add(g);
+      X[i] = g.Y;
+      level[i] += g.depth;
+    }
+    for (auto& b : pk.out) p->launches.push_back(b);
+    p->ntiles += pk.tiles;
+  }
+  p->nblocks = resident_blocks(dtype, tblock);
+  const int rc = new_queue(&p->d_queue);
+  if (rc != NPBX_OK) {
+    plan_free(p);
+    return rc;
+  }
+  *plan = p;
+  return NPBX_OK;
+}
+
+extern "C" int npbx_plan_launch(void* plan, void* stream) {
+  NPBX_REQUIRE(plan, "plan_launch: null plan");
+  Plan* p = static_cast<Plan*>(plan);
+  hipStream_t s = as_stream(stream);
+  if (p->exec) {
+    NPBX_HIP(hipGraphLaunch(p->exec, s));
+    return NPBX_OK;
+  }
+  return plan_enqueue(p, s);
+}
+
+extern "C" int npbx_plan_capture(void* plan, void* stream) {
+  NPBX_REQUIRE(plan, "plan_capture: null plan");
+  Plan* p = static_cast<Plan*>(plan);
+  NPBX_REQUIRE(stream, "plan_capture: capture needs a non-default stream");
+  hipStream_t s = as_stream(stream);
+  if (p->exec) return NPBX_OK;
+  NPBX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const int rc = plan_enqueue(p, s);
+  hipGraph_t graph = nullptr;
+  const hipError_t end = hipStreamEndCapture(s, &graph);
+  NPBX_TRY(rc);
+  NPBX_HIP(end);
+  p->graph = graph;
+  NPBX_HIP(hipGraphInstantiate(&p->exec, graph, nullptr, nullptr, 0));
+  return NPBX_OK;
+}
+
+extern "C" int npbx_plan_info(void* plan, int64_t* launches, int64_t* tiles, int64_t* lups) {
+  NPBX_REQUIRE(plan, "plan_info: null plan");
+  const Plan* p = static_cast<const Plan*>(plan);
+  if (launches) *launches = static_cast<int64_t>(p->launches.size());
+  if (tiles) *tiles = p->ntiles;
+  if (lups) *lups = p->lups;
+  return NPBX_OK;
+}
+
+extern "C" int npbx_plan_destroy(void* plan) {
+  if (!plan) return NPBX_OK;
+  // hipFree waits for work that may still use the queue state.
+  return plan_free(static_cast<Plan*>(plan));
+}
+
+// ---- serving: one round per grid per launch ---------------------------------
+extern "C" int npbx_round_depths(int64_t nsweeps, int32_t tblock, int32_t* depths,
+                                 int32_t capacity, int32_t* count) {
+  NPBX_REQUIRE(count && nsweeps >= 0 && tblock >= 2, "round_depths: bad arguments");
+  const std::vector<int> d = round_depths(nsweeps, tblock);
+  *count = static_cast<int32_t>(d.size());
+  NPBX_REQUIRE(static_cast<int64_t>(d.size()) <= capacity && (depths || d.empty()),
+               "round_depths: %zu rounds do not fit %d", d.size(), capacity);
+  for (size_t i = 0; i < d.size(); ++i) depths[i] = d[i];
+  return NPBX_OK;
+}
+
+extern "C" int npbx_queue_create(void** queue) {
+  NPBX_REQUIRE(queue, "queue_create: null output pointer");
+  QueueState* q = nullptr;
+  NPBX_TRY(new_queue(&q));
+  *queue = q;
+  return NPBX_OK;
+}
+
+extern "C" int npbx_queue_destroy(void* queue) {
+  if (queue) NPBX_HIP(hipFree(queue));
+  return NPBX_OK;
+}
+
+extern "C" int npbx_rounds_launch(const npbx_round_spec* specs, int32_t count, int32_t tblock,
+                                  int32_t seg, void* queue, void* stream) {
+  NPBX_REQUIRE(specs && count > 0 && queue, "rounds_launch: no rounds or no queue");
+  const int dtype = specs[0].dtype;
+  NPBX_REQUIRE(dtype_size(dtype) != 0, "rounds_launch: bad dtype %d", dtype);
+  const int tmax = max_depth(dtype);
+  NPBX_REQUIRE(tblock >= 2 && tblock <= tmax, "rounds_launch: tblock=%d outside [2, %d] for %s",
+               tblock, tmax, dtype_name(dtype));
+  if (seg <= 0) seg = 32;
+  Packer pk;
+  for (int i = 0; i < count; ++i) {
+    const npbx_round_spec& s = specs[i];
+    GridRound g;
+    NPBX_TRY(check_grid("rounds_launch", i, s.kind, s.dtype, s.n, s.A, s.B, dtype, g));
+    NPBX_REQUIRE((s.X == s.A || s.X == s.B) && (s.Y == s.A || s.Y == s.B) && s.X != s.Y,
+                 "rounds_launch: grid %d must read one of A, B and write the other", i);
+    NPBX_REQUIRE(s.depth >= 1 && s.depth <= tblock && s.level0 >= 0,
+                 "rounds_launch: grid %d depth %d outside [1, %d]", i, s.depth, tblock);
+    tile_grid(g, g.kind, g.n, dtype, tblock, seg);
+    g.X = s.X;
+    g.Y = s.Y;
+    g.parity0 = static_cast<int32_t>(s.level0 & 1);
+    g.depth = s.depth;
+    pk.add(g);
+  }
+  const int nblocks = resident_blocks(dtype, tblock);
+  for (const RoundBatch& b : pk.out) {
+    const int nt = b.cum[b.count];
+    NPBX_TRY(launch_any(dtype, tblock, b, static_cast<QueueState*>(queue), std::min(nt, nblocks),
+                        as_stream(stream)));
+  }
+  return NPBX_OK;
+}

@@ -22,6 +22,8 @@ from . import utilities as util
 
 
 class Test:
+    __test__ = False  # a benchmark runner, not a pytest test class
+
     def __init__(self, bench, frmwrk, npfrmwrk=None):
         self.bench = bench
         self.frmwrk = frmwrk

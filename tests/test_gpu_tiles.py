@@ -1,0 +1,172 @@
+"""Temporal-blocked and batched sweeps on the GPU.
+
+The tile kernels perform, per point, exactly the plain sweep's arithmetic,
+so every result here must be bitwise equal to the plain per-grid sweeps
+(tblock=1), whose FP64 form is itself bitwise the NumPy oracle (checked
+again here for FP64).  Sizes cover one and several tiles along every axis,
+odd and even sweep counts, both sweep parities, and depth-1 rounds.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import stencils as ref
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+DT = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16}
+TMAX = {"f64": 2, "f32": 4, "bf16": 2}
+
+
+@pytest.fixture(scope="module")
+def st():
+    from npbench_amd import device, stencils
+    device.require_device()
+    return stencils
+
+
+def grids(kind, n, dtype, seed):
+    rng = np.random.default_rng(seed)
+    shape = (n, n) if kind == "jacobi_2d" else (n, n, n)
+    A = rng.random(shape)
+    B = rng.random(shape)
+    return A, B
+
+
+def run(st, kind, A, B, dtype, nsweeps, first, tblock):
+    from npbench_amd import device
+    a, b = device.to_device(A, DT[dtype]), device.to_device(B, DT[dtype])
+    f = st.jacobi2d_sweeps if kind == "jacobi_2d" else st.heat3d_sweeps
+    f(a, b, nsweeps, first=first, tblock=tblock)
+    torch.cuda.synchronize()
+    return a, b
+
+
+def same(x, y):
+    return torch.equal(x.view(torch.int16) if x.dtype == torch.bfloat16 else x,
+                       y.view(torch.int16) if y.dtype == torch.bfloat16 else y)
+
+
+JACOBI_CASES = [("f64", 8), ("f64", 130), ("f64", 266), ("f32", 12), ("f32", 260), ("f32", 516),
+                ("bf16", 16), ("bf16", 1032)]
+HEAT_CASES = [("f64", 6), ("f64", 40), ("f64", 132), ("f32", 8), ("f32", 68), ("f32", 264),
+              ("bf16", 16), ("bf16", 520)]
+
+
+@pytest.mark.parametrize("dtype,n", JACOBI_CASES)
+@pytest.mark.parametrize("nsweeps,first", [(1, 0), (2, 1), (5, 0), (9, 1), (14, 0)])
+def test_jacobi_tiled_matches_plain(st, dtype, n, nsweeps, first):
+    A, B = grids("jacobi_2d", n, dtype, n + nsweeps)
+    pa, pb = run(st, "jacobi_2d", A, B, dtype, nsweeps, first, 1)
+    for T in range(2, TMAX[dtype] + 1):
+        print(f"jacobi tiled {dtype} n={n} sweeps={nsweeps} first={first} T={T}", flush=True)
+        ta, tb = run(st, "jacobi_2d", A, B, dtype, nsweeps, first, T)
+        assert same(ta, pa) and same(tb, pb), f"T={T}"
+    if dtype == "f64":
+        ra, rb = A.copy(), B.copy()
+        for s in range(first, first + nsweeps):
+            if s % 2 == 0:
+                ref.jacobi_sweeps(ra, rb, 1)
+            else:
+                ref.jacobi_sweeps(rb, ra, 1)
+        assert np.array_equal(pa.cpu().numpy(), ra) and np.array_equal(pb.cpu().numpy(), rb)
+
+
+@pytest.mark.parametrize("dtype,n", HEAT_CASES)
+@pytest.mark.parametrize("nsweeps,first", [(1, 1), (3, 0), (6, 1), (11, 0)])
+def test_heat_tiled_matches_plain(st, dtype, n, nsweeps, first):
+    A, B = grids("heat_3d", n, dtype, 3 * n + nsweeps)
+    pa, pb = run(st, "heat_3d", A, B, dtype, nsweeps, first, 1)
+    for T in range(2, TMAX[dtype] + 1):
+        print(f"heat tiled {dtype} n={n} sweeps={nsweeps} first={first} T={T}", flush=True)
+        ta, tb = run(st, "heat_3d", A, B, dtype, nsweeps, first, T)
+        assert same(ta, pa) and same(tb, pb), f"T={T}"
+    if dtype == "f64" and n <= 40:
+        ra, rb = A.copy(), B.copy()
+        for s in range(first, first + nsweeps):
+            if s % 2 == 0:
+                ref.heat_sweeps(ra, rb, 1)
+            else:
+                ref.heat_sweeps(rb, ra, 1)
+        assert np.array_equal(pa.cpu().numpy(), ra) and np.array_equal(pb.cpu().numpy(), rb)
+
+
+def _mixed_batch(dtype, seed):
+    """Uneven grids of both kinds with different sweep ranges."""
+    rng = np.random.default_rng(seed)
+    specs = [("jacobi_2d", 64, 7, 0), ("heat_3d", 24, 4, 1), ("jacobi_2d", 520, 12, 1),
+             ("heat_3d", 68, 9, 0), ("jacobi_2d", 8, 1, 0), ("heat_3d", 8, 2, 0),
+             ("jacobi_2d", 1024, 3, 0), ("heat_3d", 132, 5, 1)]
+    out = []
+    vec = {"f64": 2, "f32": 4, "bf16": 8}[dtype]
+    for kind, n, ns, first in specs:
+        n = -(-n // vec) * vec  # whole 16-byte vectors per row
+        A, B = grids(kind, n, dtype, int(rng.integers(1 << 30)))
+        out.append((kind, A, B, ns, first))
+    return out
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f64", "bf16"])
+def test_plan_batch_matches_each_grid_alone(st, dtype):
+    from npbench_amd import batch, device
+    cases = _mixed_batch(dtype, 5)
+    for T in range(2, TMAX[dtype] + 1):
+        print(f"plan batch {dtype} T={T}", flush=True)
+        gs = []
+        for kind, A, B, ns, first in cases:
+            a, b = device.to_device(A, DT[dtype]), device.to_device(B, DT[dtype])
+            gs.append(batch.Grid(kind, a, b, ns, first))
+        plan = batch.Plan(gs, tblock=T)
+        info = plan.info()
+        assert info["tiles"] > 0 and info["launches"] >= 2
+        plan.launch()
+        torch.cuda.synchronize()
+        for g, (kind, A, B, ns, first) in zip(gs, cases):
+            pa, pb = run(st, kind, A, B, dtype, ns, first, 1)
+            assert same(g.A, pa) and same(g.B, pb), f"{kind} n={A.shape[0]}"
+        plan.close()
+
+
+def test_plan_graph_replays_match_eager(st):
+    """Each replay of the captured graph, with eager work and syncs between
+    replays, gives the eager result (fresh inputs copied in before each)."""
+    from npbench_amd import batch, device
+    cases = _mixed_batch("f32", 9)
+    gs, inputs = [], []
+    for kind, A, B, ns, first in cases:
+        a, b = device.to_device(A, torch.float32), device.to_device(B, torch.float32)
+        inputs.append((a.clone(), b.clone()))
+        gs.append(batch.Grid(kind, a, b, ns, first))
+    plan = batch.Plan(gs, tblock=3)
+    plan.launch()
+    torch.cuda.synchronize()
+    eager = [(g.A.clone(), g.B.clone()) for g in gs]
+    side = torch.cuda.Stream()
+    for g, (a0, b0) in zip(gs, inputs):
+        g.A.copy_(a0)
+        g.B.copy_(b0)
+    torch.cuda.synchronize()
+    plan.capture(side)
+    for rep in range(3):
+        for g, (a0, b0) in zip(gs, inputs):
+            g.A.copy_(a0)
+            g.B.copy_(b0)
+        torch.cuda.synchronize()
+        plan.launch(side)
+        side.synchronize()
+        _ = torch.ones(16, device="cuda").sum().item()  # eager work between replays
+        for g, (ea, eb) in zip(gs, eager):
+            assert torch.equal(g.A, ea) and torch.equal(g.B, eb), f"replay {rep}"
+    plan.close()
+
+
+def test_plan_rejects_bad_grids(st):
+    from npbench_amd import _native, batch, device
+    a = device.to_device(np.zeros((10, 10)), torch.float32)  # 10 % 4 != 0
+    b = a.clone()
+    with pytest.raises(_native.NpbxError):
+        batch.Plan([batch.Grid("jacobi_2d", a, b, 4)], tblock=2)
+    a = device.to_device(np.zeros((12, 12)), torch.float64)
+    with pytest.raises(_native.NpbxError):
+        batch.Plan([batch.Grid("jacobi_2d", a, a.clone(), 4)], tblock=3)  # FP64 max T is 2
