@@ -1,20 +1,23 @@
-"""Benchmark of the MI355X stencil path (one GPU).
+"""Benchmark of the MI355X stencil path on one GPU.
 
-Headline workload: 3-D 7-point heat on a 256^3 grid, TSTEPS=101 (100
-ping-pong iterations = 200 sweeps), seeded uniform[0,1) FP64 inputs with
-B = A.copy(), stored on the device in FP32 (the reduced-precision storage
-form).  One "step" is one full kernel call (200 sweeps) on inputs already
-resident in HBM.  The plain FP64 drop-in (one launch per sweep) is timed on
-the same workload beside it, and the FP32 result is compared with the FP64
-result (bitwise the NumPy oracle, as the parity tests show).
+Headline ("value"): the batched FP32 form of the 3-D heat configuration --
+BATCH independent 256^3 grids, each exactly the 3-D heat configuration
+(seeded uniform[0,1) A, B = A.copy(), TSTEPS=101 = 100 ping-pong iterations
+= 200 sweeps, fixed boundaries), stored in FP32, advanced together with the
+launch shape the autotuner picks for this class (plain sweep or
+temporal-blocked tile plan).  One "step" is one full pass: every grid of the
+batch through its 200 sweeps, inputs resident in HBM.  The batch's working
+set (1 GiB) is far beyond the 256 MB Infinity Cache.
 
-Also reported (same JSON line, "kernels"): Jacobi 1000^2 FP64 (TSTEPS=101),
-hdiff 256x256x160 (FP64/FP32/BF16), vadv 256x256x160 (FP64, FP32 fields), and
-a measured device copy rate.  Device times come from HIP events recorded on
-the stream the kernels are launched on.
+Beside it, on the same 256^3 configuration: the plain FP64 drop-in (one
+launch per sweep; the comparison base of the metric), a single FP32 grid,
+and the FP32-vs-FP64 error (the FP64 GPU path is bitwise the NumPy oracle).
+Then a serving run (Poisson mix at fixed QPS: p50/p99 latency and GLUP/s),
+per-kernel lines (Jacobi 1000^2, hdiff and vadv 256x256x160, a device copy),
+and the CPU baseline (NumPy FP64 oracle on this host, bounded sample).
 
-Prints ONE JSON line: metric/value/unit, roofline of the dominant kernel and
-the CPU baseline (the NumPy FP64 oracle timed on this host, bounded sample).
+Device times come from HIP events recorded on the stream the kernels are
+launched on.  Prints ONE JSON line.
 """
 
 import argparse
@@ -36,6 +39,7 @@ HBM_PEAK_GBS = 8000.0
 HEAT_N = 256
 TSTEPS = 101
 SWEEPS = 2 * (TSTEPS - 1)
+BATCH = 8
 SIZE = {"f64": 8, "f32": 4, "bf16": 2}
 
 
@@ -54,7 +58,6 @@ class Clock:
     """Wall clock plus HIP events on the launch stream around a region."""
 
     def __init__(self, dev, stream):
-        self.dev = dev
         self.stream = stream
         self.timer = dev.Timer(stream)
 
@@ -77,45 +80,174 @@ def roofline(bytes_per_launch, launch_s, kernel):
             "kernel": kernel, "avg_launch_us": round(launch_s * 1e6, 3)}
 
 
+def seeded_grid(n):
+    """Seeded uniform[0,1) FP64 A (numpy default_rng(42)); B = A.copy()."""
+    A = np.random.default_rng(42).random((n, n, n))
+    return A, A.copy()
+
+
 # ----------------------------------------------------------------- headline
-def heat_headline(st, dev, torch, clock, args):
-    rng = np.random.default_rng(42)
-    A64 = rng.random((HEAT_N,) * 3)
-    B64 = A64.copy()
+def batched_fp32(st, dev, batch, tune, torch, clock, args, A64):
+    cfg = tune.best("heat_3d", torch.float32, HEAT_N, args.batch, cache=None,
+                    sweeps=16, verbose=False)
+    tune.apply(cfg)
+    base = dev.to_device(A64, torch.float32)
+    grids = [(base.clone(), base.clone()) for _ in range(args.batch)]
+    del base
     lup_sweep = (HEAT_N - 2) ** 3
-    lup_step = lup_sweep * SWEEPS
-    out = {}
-    for tag, dtype, tblock in (("f32", torch.float32, args.tblock), ("f64", torch.float64, 1)):
-        A = dev.to_device(A64, dtype)
-        B = dev.to_device(B64, dtype)
+    lup_step = args.batch * lup_sweep * SWEEPS
+    if cfg["T"] == 1:
+        def step():
+            for a, b in grids:
+                st.heat3d_sweeps(a, b, SWEEPS, stream=clock.stream)
+        launches = args.batch * SWEEPS
+        kernel = "heat3d_sweep_vec<float>"
+        plan = None
+    else:
+        plan = batch.Plan([batch.Grid("heat_3d", a, b, SWEEPS) for a, b in grids],
+                          tblock=cfg["T"], seg=cfg["seg"])
+        if args.graph:
+            side = torch.cuda.Stream()
+            plan.capture(side)
+            clock = type(clock)(dev, side)
+        launches = plan.info()["launches"]
+        kernel = f"stencil_tiles<float,T={cfg['T']},R={cfg['rows']}>"
 
         def step():
-            st.heat3d_sweeps(A, B, SWEEPS, tblock=tblock, stream=clock.stream)
+            plan.launch(clock.stream)
+    for _ in range(args.warmup):
+        step()
+    wall, ev = clock.run(step, args.steps)
+    # algorithmic bytes: one read and one write of every grid per round trip
+    bytes_per_lup = 2 * SIZE["f32"] / cfg["T"]
+    out = {"cfg": cfg, "wall": wall, "event": ev,
+           "glups": lup_step * args.steps / wall / 1e9,
+           "roofline": roofline(bytes_per_lup * lup_step / launches,
+                                ev / (args.steps * launches), kernel),
+           "launches_per_step": launches}
+    if plan is not None:
+        plan.close()
+    del grids
+    tune.apply({"T": 1, "shape": 0})
+    return out, lup_step
 
-        for _ in range(args.warmup):
-            step()
-        wall, ev = clock.run(step, args.steps)
-        launches = args.steps * SWEEPS if tblock <= 1 else None
-        out[tag] = {"wall": wall, "event": ev, "launches": launches,
-                    "glups": lup_step * args.steps / wall / 1e9,
-                    "glups_device": lup_step * args.steps / ev / 1e9}
-        del A, B
-    # accuracy: one step from the same initial state
+
+def single_grid(st, dev, torch, clock, args, A64, dtype, shape):
+    from npbench_amd import _native as nat
+    nat.call("npbx_set_tuning", b"sweep.shape", shape)
+    A = dev.to_device(A64, dtype)
+    B = A.clone()
+
+    def step():
+        st.heat3d_sweeps(A, B, SWEEPS, stream=clock.stream)
+    for _ in range(args.warmup):
+        step()
+    wall, ev = clock.run(step, args.steps)
+    nat.call("npbx_set_tuning", b"sweep.shape", 0)
+    lup_step = (HEAT_N - 2) ** 3 * SWEEPS
+    return {"wall": wall, "event": ev, "glups": lup_step * args.steps / wall / 1e9,
+            "launch_s": ev / (args.steps * SWEEPS)}
+
+
+def fp32_error(st, dev, torch, A64, B64):
     a32, b32 = dev.to_device(A64, torch.float32), dev.to_device(B64, torch.float32)
     a64, b64 = dev.to_device(A64), dev.to_device(B64)
-    st.heat3d_sweeps(a32, b32, SWEEPS, tblock=args.tblock)
+    st.heat3d_sweeps(a32, b32, SWEEPS)
     st.heat3d_sweeps(a64, b64, SWEEPS)
     torch.cuda.synchronize()
-    err = (a32.double() - a64).abs().max().item() / a64.abs().max().item()
-    return out, err, lup_sweep, lup_step
+    return (a32.double() - a64).abs().max().item() / a64.abs().max().item()
+
+
+# ----------------------------------------------------------------- serving
+def serving(torch, args):
+    from npbench_amd import serve
+    reqs = serve.poisson_mix(args.serve_requests, args.serve_qps, seed=42)
+    srv = serve.Server(dtype=torch.float32, tblock=2, hbm_budget=32 << 30, max_active=48)
+    srv.run(serve.poisson_mix(8, 1e6, seed=7), realtime=False)  # warm-up
+    t0 = time.perf_counter()
+    done = srv.run(reqs, realtime=True)
+    wall = time.perf_counter() - t0
+    out = serve.summarize(done, wall)
+    out.update({"qps": args.serve_qps, "mix": "50% jacobi_2d side log-U[128,4096], 50% heat_3d "
+                "side U[32,256], steps U[10,200], FP32, seed 42", "tblock": 2,
+                "peak_active": srv.peak_active, "ticks": srv.ticks})
+    srv.close()
+    return out
+
+
+# ----------------------------------------------------------------- kernels
+def kernel_suite(st, dev, torch, clock, reps, copy_only=False):
+    res = {}
+    rng = np.random.default_rng(42)
+    from npbench_amd import _native as nat
+
+    nbytes = 1 << 31
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+
+    def cp():
+        nat.call("npbx_stream_copy", dst.data_ptr(), src.data_ptr(), nbytes,
+                 dev.stream_handle(clock.stream))
+    cp()
+    _, ev = clock.run(cp, reps)
+    res["stream_copy_2GiB"] = {"GBps": round(2 * nbytes / (ev / reps) / 1e9, 1),
+                               "bytes_per_launch": 2 * nbytes}
+    del src, dst
+    if copy_only:
+        return res
+
+    n = 1000
+    A = dev.to_device(rng.random((n, n)))
+    B = dev.to_device(rng.random((n, n)))
+
+    def jac():
+        st.jacobi2d_sweeps(A, B, SWEEPS, stream=clock.stream)
+    jac()
+    wall, ev = clock.run(jac, reps)
+    lups = (n - 2) ** 2 * SWEEPS
+    res["jacobi2d_1000_f64"] = {"glups": round(lups * reps / wall / 1e9, 2),
+                                "avg_sweep_us": round(ev / reps / SWEEPS * 1e6, 3),
+                                "hbm_frac": round(16 * lups * reps / ev / 1e9 / HBM_PEAK_GBS, 4)}
+    del A, B
+
+    I, J, K = 256, 256, 160
+    inf = rng.random((I + 4, J + 4, K))
+    coeff = 0.25 * rng.random((I, J, K))
+    for tag, dt in (("f64", torch.float64), ("f32", torch.float32), ("bf16", torch.bfloat16)):
+        di, dc = dev.to_device(inf, dt), dev.to_device(coeff, dt)
+        do = torch.empty((I, J, K), dtype=dt, device="cuda")
+
+        def hd():
+            st.hdiff(di, do, dc, stream=clock.stream)
+        hd()
+        _, ev = clock.run(hd, reps)
+        byts = SIZE[tag] * ((I + 4) * (J + 4) * K + 2 * I * J * K)
+        res[f"hdiff_256x256x160_{tag}"] = {
+            "glups": round(I * J * K * reps / ev / 1e9, 2), "avg_us": round(ev / reps * 1e6, 3),
+            "hbm_frac": round(byts * reps / ev / 1e9 / HBM_PEAK_GBS, 4)}
+        del di, dc, do
+
+    fields = [rng.random((I, J, K)), rng.random((I, J, K)), 2 * rng.random((I + 1, J, K)) - 1,
+              rng.random((I, J, K)), rng.random((I, J, K))]
+    for tag, dt in (("f64", torch.float64), ("f32", torch.float32)):
+        d = [dev.to_device(a, dt) for a in fields]
+
+        def va():
+            st.vadv(*d, 0.15, stream=clock.stream)
+        va()
+        _, ev = clock.run(va, reps)
+        byts = SIZE[tag] * (5 * I * J * K + J * K + I * J * K)
+        res[f"vadv_256x256x160_{tag}"] = {
+            "glups": round(I * J * K * reps / ev / 1e9, 2), "avg_us": round(ev / reps * 1e6, 3),
+            "hbm_frac": round(byts * reps / ev / 1e9 / HBM_PEAK_GBS, 4)}
+        del d
+    return res
 
 
 def cpu_baseline(n, sweeps):
     """NumPy FP64 oracle, median time per heat sweep on this host (1 core)."""
     from oracle import stencils as ref
-    rng = np.random.default_rng(42)
-    A = rng.random((n, n, n))
-    B = A.copy()
+    A, B = seeded_grid(n)
     times = []
     for s in range(sweeps):
         t0 = time.perf_counter()
@@ -136,79 +268,38 @@ def cpu_baseline(n, sweeps):
     }
 
 
-# ----------------------------------------------------------------- kernels
-def kernel_suite(st, dev, torch, clock, reps):
-    res = {}
-    rng = np.random.default_rng(42)
-
-    # copy rate (2 GiB) for reference against the datasheet peak
-    nbytes = 1 << 31
-    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
-    dst = torch.empty_like(src)
-    from npbench_amd import _native as nat
-    cp = lambda: nat.call("npbx_stream_copy", dst.data_ptr(), src.data_ptr(), nbytes,  # noqa
-                          dev.stream_handle(clock.stream))
-    cp()
-    _, ev = clock.run(cp, reps)
-    res["stream_copy_2GiB"] = {"GBps": round(2 * nbytes / (ev / reps) / 1e9, 1)}
-    del src, dst
-
-    # Jacobi 1000^2 FP64, TSTEPS=101 (plain sweeps)
-    n = 1000
-    A = dev.to_device(rng.random((n, n)))
-    B = dev.to_device(rng.random((n, n)))
-    f = lambda: st.jacobi2d_sweeps(A, B, SWEEPS, stream=clock.stream)  # noqa
-    f()
-    wall, ev = clock.run(f, reps)
-    lups = (n - 2) ** 2 * SWEEPS
-    res["jacobi2d_1000_f64"] = {"glups": round(lups * reps / wall / 1e9, 2),
-                                "avg_sweep_us": round(ev / reps / SWEEPS * 1e6, 3),
-                                "hbm_frac": round(16 * lups * reps / ev / 1e9 / HBM_PEAK_GBS, 4)}
-    del A, B
-
-    # hdiff 256x256x160
-    I, J, K = 256, 256, 160
-    inf = rng.random((I + 4, J + 4, K))
-    coeff = 0.25 * rng.random((I, J, K))
-    for tag, dt in (("f64", torch.float64), ("f32", torch.float32), ("bf16", torch.bfloat16)):
-        di, dc = dev.to_device(inf, dt), dev.to_device(coeff, dt)
-        do = torch.empty((I, J, K), dtype=dt, device="cuda")
-        f = lambda: st.hdiff(di, do, dc, stream=clock.stream)  # noqa
-        f()
-        wall, ev = clock.run(f, reps)
-        byts = SIZE[tag] * ((I + 4) * (J + 4) * K + 2 * I * J * K)
-        res[f"hdiff_256x256x160_{tag}"] = {
-            "glups": round(I * J * K * reps / ev / 1e9, 2),
-            "avg_us": round(ev / reps * 1e6, 3),
-            "hbm_frac": round(byts * reps / ev / 1e9 / HBM_PEAK_GBS, 4)}
-        del di, dc, do
-
-    # vadv 256x256x160
-    fields = [rng.random((I, J, K)), rng.random((I, J, K)), 2 * rng.random((I + 1, J, K)) - 1,
-              rng.random((I, J, K)), rng.random((I, J, K))]
-    for tag, dt in (("f64", torch.float64), ("f32", torch.float32)):
-        d = [dev.to_device(a, dt) for a in fields]
-        f = lambda: st.vadv(*d, 0.15, stream=clock.stream)  # noqa
-        f()
-        wall, ev = clock.run(f, reps)
-        byts = SIZE[tag] * (5 * I * J * K + J * K + I * J * K)
-        res[f"vadv_256x256x160_{tag}"] = {
-            "glups": round(I * J * K * reps / ev / 1e9, 2),
-            "avg_us": round(ev / reps * 1e6, 3),
-            "hbm_frac": round(byts * reps / ev / 1e9 / HBM_PEAK_GBS, 4)}
-        del d
-    return res
+def only(args, st, dev, batch, tune, torch, clock, A64):
+    """One part of the benchmark on its own (for profiler runs)."""
+    if args.only == "headline":
+        head, _ = batched_fp32(st, dev, batch, tune, torch, clock, args, A64)
+        return {"part": "headline", "glups": head["glups"], "cfg": head["cfg"],
+                "roofline": head["roofline"]}
+    if args.only in ("fp64", "fp32"):
+        dt = torch.float64 if args.only == "fp64" else torch.float32
+        r = single_grid(st, dev, torch, clock, args, A64, dt, 0)
+        return {"part": args.only, "glups": r["glups"], "avg_launch_us": r["launch_s"] * 1e6}
+    if args.only == "serve":
+        return {"part": "serve", **serving(torch, args)}
+    if args.only == "copy":
+        return {"part": "copy", **kernel_suite(st, dev, torch, clock, args.steps, copy_only=True)}
+    return {"part": "kernels", **kernel_suite(st, dev, torch, clock, args.steps)}
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--tblock", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=BATCH)
+    ap.add_argument("--graph", type=int, default=1, help="replay a captured HIP graph (1/0)")
     ap.add_argument("--cpu-sweeps", type=int, default=64,
                     help="sweeps in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-qps", type=float, default=400.0)
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel lines")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
+    ap.add_argument("--only", choices=["headline", "fp64", "fp32", "copy", "kernels", "serve"],
+                    help="run one part and print its figures (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -216,41 +307,55 @@ def main():
         sys.exit(2)
 
     import torch
-    from npbench_amd import device as dev, stencils as st
+    from npbench_amd import batch, device as dev, stencils as st, tune
     dev.require_device(0)
     clock = Clock(dev, torch.cuda.current_stream())
+    A64, B64 = seeded_grid(HEAT_N)
+    if args.only:
+        print(json.dumps(only(args, st, dev, batch, tune, torch, clock, A64)), flush=True)
+        return
 
-    heat, err, lup_sweep, lup_step = heat_headline(st, dev, torch, clock, args)
-    h32, h64 = heat["f32"], heat["f64"]
-    bpl32 = 8.0 / max(args.tblock, 1)
+    head, lup_step = batched_fp32(st, dev, batch, tune, torch, clock, args, A64)
+    f64 = single_grid(st, dev, torch, clock, args, A64, torch.float64, 0)
+    f32 = single_grid(st, dev, torch, clock, args, A64, torch.float32, 0)
+    err = fp32_error(st, dev, torch, A64, B64)
+    lup_sweep = (HEAT_N - 2) ** 3
+
     line = {
         "metric": METRIC,
-        "value": round(h32["glups"], 3),
+        "value": round(head["glups"], 3),
         "unit": "GLUP/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(h32["wall"] / args.steps * 1e3, 4),
+        "ms_per_step": round(head["wall"] / args.steps * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded uniform[0,1), numpy default_rng(42))",
-        "config": {"workload": "heat_3d 256^3, TSTEPS=101 (200 sweeps), FP32 storage",
-                   "N": HEAT_N, "tsteps": TSTEPS, "sweeps_per_step": SWEEPS,
-                   "storage": "f32", "tblock": args.tblock},
-        "roofline": roofline(bpl32 * lup_sweep, h32["event"] / (args.steps * SWEEPS),
-                             "heat3d_sweep_vec<float>"),
+        "config": {"workload": f"heat_3d 256^3 x {args.batch} grids, TSTEPS=101 (200 sweeps "
+                               "each), FP32 storage, batched",
+                   "N": HEAT_N, "tsteps": TSTEPS, "sweeps_per_grid": SWEEPS,
+                   "batch": args.batch, "storage": "f32",
+                   "shape": head["cfg"], "graph": bool(args.graph and head["cfg"]["T"] > 1),
+                   "launches_per_step": head["launches_per_step"]},
+        "roofline": head["roofline"],
         "fp64_plain": {
-            "value": round(h64["glups"], 3), "unit": "GLUP/s",
-            "ms_per_step": round(h64["wall"] / args.steps * 1e3, 4),
-            "roofline": roofline(16.0 * lup_sweep, h64["event"] / (args.steps * SWEEPS),
-                                 "heat3d_sweep_vec<double>")},
-        "speedup_vs_fp64_plain": round(h32["glups"] / h64["glups"], 3),
+            "value": round(f64["glups"], 3), "unit": "GLUP/s",
+            "workload": "heat_3d 256^3, TSTEPS=101, FP64 drop-in, one launch per sweep",
+            "ms_per_step": round(f64["wall"] / args.steps * 1e3, 4),
+            "roofline": roofline(16.0 * lup_sweep, f64["launch_s"], "heat3d_sweep_vec<double>")},
+        "fp32_single_grid": {
+            "value": round(f32["glups"], 3), "unit": "GLUP/s",
+            "roofline": roofline(8.0 * lup_sweep, f32["launch_s"], "heat3d_sweep_vec<float>")},
+        "speedup_vs_fp64_plain": round(head["glups"] / f64["glups"], 3),
         "max_rel_err_vs_fp64": err,
     }
+    if not args.no_serve:
+        line["serving"] = serving(torch, args)
     if not args.no_kernels:
-        line["kernels"] = kernel_suite(st, dev, torch, clock, max(3, args.steps // 2))
+        line["kernels"] = kernel_suite(st, dev, torch, clock, 5)
     if args.cpu_sweeps > 0:
         line["cpu_baseline"] = cpu_baseline(HEAT_N, args.cpu_sweeps)
     print(json.dumps(line), flush=True)

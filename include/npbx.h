@@ -174,6 +174,15 @@ int npbx_queue_destroy(void *queue);
 int npbx_rounds_launch(const npbx_round_spec *rounds, int32_t count,
                        int32_t tblock, int32_t seg, void *queue, void *stream);
 
+/* ---- tuning ------------------------------------------------------------------
+ * Process-wide knobs read when tile launches are planned (the autotuner in
+ * npbench_amd/tune.py sets them): "heat.rows" = rows per wave of a heat tile
+ * (2, 4, 6, 8; built for T=2: 2..8 (FP32), 2..6 (FP64), 2 and 4 (BF16); T=3:
+ * 4 and 6; T=4: 4), "heat.seg" =
+ * default planes per heat tile.  Results do not depend on them. */
+int npbx_set_tuning(const char *key, int64_t value);
+int npbx_get_tuning(const char *key, int64_t *value);
+
 /* ---- horizontal diffusion (hdiff_numpy.py:5-29) ----------------------------
  * in_field: (I+4, J+4, K); out_field, coeff: (I, J, K).  nsteps == 1 is the
  * reference; nsteps > 1 iterates, feeding out back into the interior of the

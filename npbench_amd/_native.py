@@ -83,6 +83,8 @@ SIGNATURES = {
     "npbx_queue_destroy": [_vp],
     "npbx_rounds_launch": [ctypes.POINTER(RoundSpec), ctypes.c_int32, ctypes.c_int32,
                            ctypes.c_int32, _vp, _vp],
+    "npbx_set_tuning": [ctypes.c_char_p, _i64],
+    "npbx_get_tuning": [ctypes.c_char_p, ctypes.POINTER(_i64)],
     "npbx_hdiff": [_int, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _vp],
     "npbx_vadv": [_int, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _i64, _i64, _i64, _vp],
 }

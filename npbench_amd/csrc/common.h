@@ -117,6 +117,11 @@ int dispatch_dtype(int dtype, F&& f) {
 int tiled_sweeps(int kind, int dtype, void* A, void* B, int n, int64_t first, int64_t nsweeps,
                  int T, int seg, hipStream_t stream);
 
+// Plain-sweep shape chosen by the tuning knob "sweep.shape" (0 = 16 planes
+// per workgroup with loads of 4 planes in flight, 1 = 16/8, 2 = 32/8,
+// 3 = 8/4).
+int sweep_shape();
+
 // Per-device scratch owned by the library (grow-only, freed at exit).  Used
 // by kernels that need storage beyond their inputs and outputs.
 int workspace(size_t bytes, void** ptr);

@@ -6,11 +6,21 @@
 // The stencil acts in (i, j) only; every k is independent.  Output point
 // (x, y, k) sits over in-field point (x+2, y+2, k).
 //
-// Kernel: lanes run along k (coalesced), a 256-thread workgroup is 64 (k) x
-// 4 (y) and grid z runs over output rows x; each thread computes one output
-// point.  The Laplacian of the centre and of its four neighbours is rebuilt
-// from cached loads (13 in-field points); the limiter keeps the reference
-// test "r * grad > 0 -> 0".
+// Marching kernel (hdiff_march): a lane owns one output column y and one
+// 16-byte vector of V consecutive k (lanes enumerate (y, k-vector) pairs,
+// so any K maps densely onto waves and loads stay contiguous), and walks
+// down a strip of kStrip output rows x.  It keeps in registers the in-field
+// rows p-2..p+2 of its column q, rows p-1..p+1 of columns q+-1 and row p of
+// q+-2 (p = x+2, q = y+2), and carries lap(p-1), lap(p) and the i-flux of
+// the face below from one row to the next, so each output point loads five
+// new vectors and computes three Laplacians and three fluxes.  The next
+// row's loads are issued before the current row is computed.  The 2-cell
+// halo of the input makes every access in bounds.  Same operations in the
+// same order as the reference, so FP64 is bitwise NumPy.
+//
+// Fallback (hdiff_kernel, K not a multiple of V or unaligned arrays): one
+// output point per thread, the Laplacians rebuilt from 13 cached loads.
+// The limiter keeps the reference test "r * grad > 0 -> 0".
 //
 // Iterated form (nsteps > 1, build-defined): step n reads the previous
 // step's output as the interior of the input field, with the 2-cell halo of
@@ -19,6 +29,7 @@
 #include <algorithm>
 
 #include "common.h"
+#include "vecio.h"
 
 namespace npbx {
 namespace {
@@ -102,19 +113,145 @@ __global__ __launch_bounds__(256) void hdiff_copy_halo(const S* __restrict__ src
   }
 }
 
+constexpr int kStrip = 16;  // output rows per thread
+
+template <class S>
+__global__ __launch_bounds__(256) void hdiff_march(const S* __restrict__ in, S* __restrict__ out,
+                                                   const S* __restrict__ coeff, int I, int J,
+                                                   int K, int xblocks, S* __restrict__ halo_dst) {
+  using T = Storage<S>;
+  using C = typename T::C;
+  constexpr int V = Vec16<S>::N;
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const int xb = b % xblocks, strip = b / xblocks;
+  const int kvs = K / V;
+  const int t = xb * 256 + threadIdx.x;
+  if (t >= J * kvs) return;
+  const int y = t / kvs, k0 = (t % kvs) * V;
+  const int x0 = strip * kStrip, x1 = min(x0 + kStrip, I);
+  const int64_t si = static_cast<int64_t>(J + 4) * K;  // in-field row stride
+  const int64_t sj = K;
+  const S* f = in + (y + 2) * sj + k0;                // column q = y+2, row 0
+  auto ld = [&](int p, int dq, C (&v)[V]) { load16(f + p * si + dq * sj, v); };
+
+  // window at row p = x0+2: c0[0..4] = rows p-2..p+2 (col q), cm/cp[0..2] =
+  // rows p-1..p+1 (cols q-1 / q+1), cmm / cpp = row p (cols q-2 / q+2)
+  C c0[5][V], cm[3][V], cp[3][V], cmm[V], cpp[V];
+  const int p0 = x0 + 2;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) ld(p0 - 2 + r, 0, c0[r]);
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    ld(p0 - 1 + r, -1, cm[r]);
+    ld(p0 - 1 + r, 1, cp[r]);
+  }
+  // the loads that complete the window of row p0 (and start each next row)
+  C n0[V], nm[V], np[V], nmm[V], npp[V], cf[V];
+  ld(p0 + 2, 0, n0);
+  ld(p0 + 1, -1, nm);
+  ld(p0 + 1, 1, np);
+  ld(p0, -2, nmm);
+  ld(p0, 2, npp);
+  const S* cfp = coeff + (static_cast<int64_t>(x0) * J + y) * K + k0;
+  load16(cfp, cf);
+
+  auto lap = [](C c, C ip, C im, C jp, C jm) {
+    C nb = ip + im;
+    nb = nb + jp;
+    nb = nb + jm;
+    return static_cast<C>(4.0) * c - nb;
+  };
+  C L0[V], flx_lo[V];  // carried: lap(p) and the flux across (p-1 | p)
+  for (int x = x0; x < x1; ++x) {
+    const int p = x + 2;
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      c0[4][v] = n0[v];
+      cm[2][v] = nm[v];
+      cp[2][v] = np[v];
+      cmm[v] = nmm[v];
+      cpp[v] = npp[v];
+    }
+    C cfx[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) cfx[v] = cf[v];
+    if (x + 1 < x1) {  // next row's loads, in flight while this row is computed
+      ld(p + 3, 0, n0);
+      ld(p + 2, -1, nm);
+      ld(p + 2, 1, np);
+      ld(p + 1, -2, nmm);
+      ld(p + 1, 2, npp);
+      load16(cfp + static_cast<int64_t>(x + 1 - x0) * J * K, cf);
+    }
+    C res[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const C c = c0[2][v];
+      if (x == x0) {
+        L0[v] = lap(c, c0[3][v], c0[1][v], cp[1][v], cm[1][v]);
+        const C Lim = lap(c0[1][v], c, c0[0][v], cp[0][v], cm[0][v]);
+        flx_lo[v] = limit<C>(L0[v] - Lim, c - c0[1][v]);
+      }
+      const C Lip = lap(c0[3][v], c0[4][v], c, cp[2][v], cm[2][v]);
+      const C Ljp = lap(cp[1][v], cp[2][v], cp[0][v], cpp[v], c);
+      const C Ljm = lap(cm[1][v], cm[2][v], cm[0][v], c, cmm[v]);
+      const C flx_hi = limit<C>(Lip - L0[v], c0[3][v] - c);
+      const C fly_hi = limit<C>(Ljp - L0[v], cp[1][v] - c);
+      const C fly_lo = limit<C>(L0[v] - Ljm, c - cm[1][v]);
+      C dv = flx_hi - flx_lo[v];
+      dv = dv + fly_hi;
+      dv = dv - fly_lo;
+      res[v] = c - cfx[v] * dv;
+      // carry to row p+1
+      L0[v] = Lip;
+      flx_lo[v] = flx_hi;
+    }
+    C o[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) o[v] = res[v];
+    if (halo_dst)
+      store16(halo_dst + p * si + (y + 2) * sj + k0, o);
+    else
+      store16(out + (static_cast<int64_t>(x) * J + y) * K + k0, o);
+    // slide the window down one row
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) c0[r][v] = c0[r + 1][v];
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        cm[r][v] = cm[r + 1][v];
+        cp[r][v] = cp[r + 1][v];
+      }
+    }
+  }
+}
+
 template <class S>
 int run_hdiff(const S* in, S* out, const S* coeff, int I, int J, int K, int64_t nsteps,
               hipStream_t stream) {
-  const int64_t si = static_cast<int64_t>(J + 4) * K, sj = K;
-  const dim3 block(kLanes, kCols);
-  const dim3 grid(static_cast<unsigned>(cdiv(K, kLanes)), static_cast<unsigned>(cdiv(J, kCols)),
-                  static_cast<unsigned>(I));
-  if (nsteps == 1) {
-    hipLaunchKernelGGL(hdiff_kernel<S>, grid, block, 0, stream, in, out, coeff, I, J, K, si, sj,
-                       static_cast<S*>(nullptr));
-    NPBX_LAUNCHED("hdiff_kernel");
+  constexpr int V = Vec16<S>::N;
+  const bool vec = K % V == 0 && aligned16(in) && aligned16(out) && aligned16(coeff);
+  const int xblocks = static_cast<int>(cdiv(static_cast<int64_t>(J) * (K / V), 256));
+  const int64_t mblocks = int64_t(xblocks) * cdiv(I, kStrip);
+  NPBX_REQUIRE(!vec || mblocks < (int64_t(1) << 31), "hdiff: grid too large");
+  auto launch = [&](const S* src, S* next) -> int {
+    if (vec) {
+      hipLaunchKernelGGL(hdiff_march<S>, dim3(static_cast<unsigned>(mblocks)), dim3(256), 0,
+                         stream, src, out, coeff, I, J, K, xblocks, next);
+      NPBX_LAUNCHED("hdiff_march");
+    } else {
+      const int64_t si = static_cast<int64_t>(J + 4) * K, sj = K;
+      const dim3 block(kLanes, kCols);
+      const dim3 grid(static_cast<unsigned>(cdiv(K, kLanes)),
+                      static_cast<unsigned>(cdiv(J, kCols)), static_cast<unsigned>(I));
+      hipLaunchKernelGGL(hdiff_kernel<S>, grid, block, 0, stream, src, out, coeff, I, J, K, si,
+                         sj, next);
+      NPBX_LAUNCHED("hdiff_kernel");
+    }
     return NPBX_OK;
-  }
+  };
+  if (nsteps == 1) return launch(in, nullptr);
   // Two work fields ping-pong; each carries the original halo.
   const size_t field = static_cast<size_t>(I + 4) * (J + 4) * K;
   void* ws = nullptr;
@@ -130,9 +267,7 @@ int run_hdiff(const S* in, S* out, const S* coeff, int I, int J, int K, int64_t 
   for (int64_t n = 0; n < nsteps; ++n) {
     const bool last = n + 1 == nsteps;
     S* next = w[n % 2];
-    hipLaunchKernelGGL(hdiff_kernel<S>, grid, block, 0, stream, src, out, coeff, I, J, K, si, sj,
-                       last ? static_cast<S*>(nullptr) : next);
-    NPBX_LAUNCHED("hdiff_kernel");
+    NPBX_TRY(launch(src, last ? nullptr : next));
     src = next;
   }
   return NPBX_OK;

@@ -15,9 +15,11 @@
 //
 // Scalar form (rows not 16-byte aligned): one k per lane, same marching.
 //
-// Arithmetic follows the reference: per axis 0.125*((x[+1] - 2*c) + x[-1]),
-// the axis terms summed i, j, k left to right, then + c.
+// Arithmetic (points.h): the FP64 drop-in follows the reference order, per
+// axis 0.125*((x[+1] - 2*c) + x[-1]), the axis terms summed i, j, k left to
+// right, then + c; FP32 arithmetic uses the 7-operation form of the same map.
 #include "common.h"
+#include "points.h"
 #include "vecio.h"
 
 namespace npbx {
@@ -26,16 +28,6 @@ namespace {
 constexpr int kLanes = 64;
 constexpr int kCols = 4;
 constexpr int kRows = 16;
-
-template <class C>
-__device__ __forceinline__ C heat_point(C c, C ip, C im, C jp, C jm, C kp, C km) {
-  const C c2 = static_cast<C>(2.0) * c;
-  const C q = static_cast<C>(0.125);
-  const C ti = q * ((ip - c2) + im);
-  const C tj = q * ((jp - c2) + jm);
-  const C tk = q * ((kp - c2) + km);
-  return ((ti + tj) + tk) + c;
-}
 
 template <class S>
 __global__ __launch_bounds__(kLanes* kCols) void heat3d_sweep(const S* __restrict__ src,
@@ -63,10 +55,10 @@ __global__ __launch_bounds__(kLanes* kCols) void heat3d_sweep(const S* __restric
   }
 }
 
-constexpr int kPlanes = 16;  // planes per workgroup (i segment)
-constexpr int kUnroll = 4;   // planes whose loads are issued together
-
-template <class S>
+// kPlanes = planes per workgroup (i segment); kUnroll = planes whose loads
+// are issued together.  Both are picked per call from the tuning knob
+// "sweep.shape" (sweep_shape below).
+template <class S, int kPlanes, int kUnroll>
 __global__ __launch_bounds__(kLanes* kCols) void heat3d_sweep_vec(const S* __restrict__ src,
                                                                  S* __restrict__ dst, int n,
                                                                  int tiles_k, int tiles_j) {
@@ -129,31 +121,44 @@ __global__ __launch_bounds__(kLanes* kCols) void heat3d_sweep_vec(const S* __res
   }
 }
 
-template <class S>
-int run_sweeps(S* A, S* B, int n, int64_t first, int64_t nsweeps, hipStream_t stream) {
+template <class S, int kPlanes, int kUnroll>
+int run_vec(S* A, S* B, int n, int64_t first, int64_t nsweeps, hipStream_t stream) {
   constexpr int V = Vec16<S>::N;
   const dim3 block(kLanes, kCols);
-  const bool vec = n % V == 0 && aligned16(A) && aligned16(B);
   const int tiles_k = static_cast<int>(cdiv(n, kLanes * V));
   const int tiles_j = static_cast<int>(cdiv(n - 2, kCols));
   const int tiles_i = static_cast<int>(cdiv(n - 2, kPlanes));
   const int64_t nblocks = int64_t(tiles_k) * tiles_j * tiles_i;
   NPBX_REQUIRE(nblocks < (int64_t(1) << 31), "heat3d: grid too large");
+  for (int64_t s = first; s < first + nsweeps; ++s) {
+    const bool even = (s % 2) == 0;
+    hipLaunchKernelGGL((heat3d_sweep_vec<S, kPlanes, kUnroll>),
+                       dim3(static_cast<unsigned>(nblocks)), block, 0, stream, even ? A : B,
+                       even ? B : A, n, tiles_k, tiles_j);
+    NPBX_LAUNCHED("heat3d_sweep_vec");
+  }
+  return NPBX_OK;
+}
+
+template <class S>
+int run_sweeps(S* A, S* B, int n, int64_t first, int64_t nsweeps, hipStream_t stream) {
+  constexpr int V = Vec16<S>::N;
+  if (n % V == 0 && aligned16(A) && aligned16(B)) {
+    switch (sweep_shape()) {
+      case 1: return run_vec<S, 16, 8>(A, B, n, first, nsweeps, stream);
+      case 2: return run_vec<S, 32, 8>(A, B, n, first, nsweeps, stream);
+      case 3: return run_vec<S, 8, 4>(A, B, n, first, nsweeps, stream);
+      default: return run_vec<S, 16, 4>(A, B, n, first, nsweeps, stream);
+    }
+  }
+  const dim3 block(kLanes, kCols);
   const dim3 grid(static_cast<unsigned>(cdiv(n - 2, kLanes)),
                   static_cast<unsigned>(cdiv(n - 2, kCols)),
                   static_cast<unsigned>(cdiv(n - 2, kRows)));
   for (int64_t s = first; s < first + nsweeps; ++s) {
     const bool even = (s % 2) == 0;
-    const S* src = even ? A : B;
-    S* dst = even ? B : A;
-    if (vec) {
-      hipLaunchKernelGGL(heat3d_sweep_vec<S>, dim3(static_cast<unsigned>(nblocks)), block, 0,
-                         stream, src, dst, n, tiles_k, tiles_j);
-      NPBX_LAUNCHED("heat3d_sweep_vec");
-    } else {
-      hipLaunchKernelGGL(heat3d_sweep<S>, grid, block, 0, stream, src, dst, n);
-      NPBX_LAUNCHED("heat3d_sweep");
-    }
+    hipLaunchKernelGGL(heat3d_sweep<S>, grid, block, 0, stream, even ? A : B, even ? B : A, n);
+    NPBX_LAUNCHED("heat3d_sweep");
   }
   return NPBX_OK;
 }

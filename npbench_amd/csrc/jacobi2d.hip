@@ -17,6 +17,7 @@
 // with W=[i,j-1], E=[i,j+1], S=[i+1,j], N=[i-1,j]; the library is compiled
 // with -ffp-contract=off so the FP64 path is bitwise equal to NumPy.
 #include "common.h"
+#include "points.h"
 #include "vecio.h"
 
 namespace npbx {
@@ -42,11 +43,7 @@ __global__ __launch_bounds__(kLanes* kStrips) void jacobi2d_sweep(const S* __res
     const C south = T::load(src[row + n + j]);
     const C west = T::load(src[row + j - 1]);
     const C east = T::load(src[row + j + 1]);
-    C acc = centre + west;
-    acc = acc + east;
-    acc = acc + south;
-    acc = acc + north;
-    dst[row + j] = T::store(static_cast<C>(0.2) * acc);
+    dst[row + j] = T::store(jacobi_point<C>(centre, west, east, south, north));
     north = centre;
     centre = south;
   }
@@ -92,11 +89,7 @@ __global__ __launch_bounds__(kLanes* kStrips) void jacobi2d_sweep_vec(const S* _
       for (int v = 0; v < V; ++v) {
         const C west = v == 0 ? w[u] : centre[v - 1];
         const C east = v == V - 1 ? e[u] : centre[v + 1];
-        C acc = centre[v] + west;
-        acc = acc + east;
-        acc = acc + south[u][v];
-        acc = acc + north[v];
-        out[v] = static_cast<C>(0.2) * acc;
+        out[v] = jacobi_point<C>(centre[v], west, east, south[u][v], north[v]);
       }
       S* o = dst + static_cast<int64_t>(i + u) * n + j0;
       if (full) {

@@ -25,7 +25,7 @@ struct GridRound {
   int32_t tiles_k;  // tiles along the contiguous axis
   int32_t tiles_j;  // tiles along the row axis
   int32_t seg;      // heat: output planes per tile along i
-  int32_t pad;
+  int32_t rows;     // heat: rows per wave (a tile is 4*rows rows)
 };
 
 // Up to kMaxBatch grid rounds passed by value as one launch's kernel

@@ -20,8 +20,9 @@
 // an output segment of `seg` planes plus T planes of halo on each side.
 // Level l keeps a three-plane register window (i-1, i, i+1); each step loads
 // one new level-0 plane and pushes it through the T levels (2.5-D spatial
-// plus temporal blocking).  Only level-0 planes are read from HBM and only
-// level-T planes of the output segment are written.
+// plus temporal blocking); the plane for step P+1 is loaded during step P.
+// Only level-0 planes are read from HBM and only level-T planes of the
+// output segment are written.
 //
 // jacobi (2-D): the tile is 4R rows x KT columns held in registers; the T
 // levels are computed in place with an LDS edge-row exchange per level.
@@ -34,10 +35,12 @@
 // the last workgroup to find the queue empty resets it, so consecutive
 // launches and graph replays need no memset and no device-side tile list.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <vector>
 
 #include "common.h"
+#include "points.h"
 #include "tiles.h"
 #include "vecio.h"
 
@@ -46,7 +49,8 @@ namespace {
 
 constexpr int kWaves = 4;
 constexpr int kThreads = 64 * kWaves;
-constexpr int kRowsHeat = 4;  // rows per wave, heat
+// Heat tiles: R rows per wave (a tile is 4R rows) is a tuning choice (see
+// npbx_set_tuning "heat.rows"); with_shape lists the instantiated (T, R).
 
 // Rows per wave of a Jacobi tile (R*V <= 64 so a lane's points fit one
 // 64-bit fixed-point mask).
@@ -58,25 +62,6 @@ template <class S>
 struct MaxDepth {
   static constexpr int value = sizeof(S) == 4 ? 4 : 2;
 };
-
-template <class C>
-__device__ __forceinline__ C heat_point(C c, C ip, C im, C jp, C jm, C kp, C km) {
-  const C c2 = static_cast<C>(2.0) * c;
-  const C q = static_cast<C>(0.125);
-  const C ti = q * ((ip - c2) + im);
-  const C tj = q * ((jp - c2) + jm);
-  const C tk = q * ((kp - c2) + km);
-  return ((ti + tj) + tk) + c;
-}
-
-template <class C>
-__device__ __forceinline__ C jacobi_point(C c, C w, C e, C s, C n) {
-  C acc = c + w;
-  acc = acc + e;
-  acc = acc + s;
-  acc = acc + n;
-  return static_cast<C>(0.2) * acc;
-}
 
 // Boundary source of a level: A for even levels, B for odd.  Selecting
 // between two pointer values (not indexing the descriptor) keeps the
@@ -102,13 +87,12 @@ __device__ __forceinline__ void col_range(int n, int tiles_k, int tk, int& kl, i
 }
 
 // ---------------------------------------------------------------- heat 3-D
-template <class S, int T>
+template <class S, int T, int R>
 __device__ __forceinline__ void heat_tile(const GridRound g, int tile,
                                           typename Storage<S>::C* lds) {
   using St = Storage<S>;
   using C = typename St::C;
   constexpr int V = Vec16<S>::N;
-  constexpr int R = kRowsHeat;
   constexpr int KT = 64 * V;
   constexpr int JT = kWaves * R;
   const int n = g.n, d = g.depth;
@@ -134,17 +118,32 @@ __device__ __forceinline__ void heat_tile(const GridRound g, int tile,
   bool row_live[R];
   uint32_t fixed = 0;  // bit r*V+v: global boundary in j or k
   int64_t rowoff[R];
+  int kb = -1, jb = -1, nkb = 0, njb = 0;  // the lane's boundary column / row
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int j = jl + w * R + r;
     row_live[r] = vlive && j >= 0 && j < n;
     rowoff[r] = static_cast<int64_t>(j) * n + k0;
+    if (row_live[r] && (j == 0 || j == n - 1)) {
+      jb = r;
+      ++njb;
+    }
 #pragma unroll
     for (int v = 0; v < V; ++v) {
       const int k = k0 + v;
       if (row_live[r] && (j == 0 || j == n - 1 || k == 0 || k == n - 1)) fixed |= 1u << (r * V + v);
     }
   }
+#pragma unroll
+  for (int v = 0; v < V; ++v)
+    if (vlive && (k0 + v == 0 || k0 + v == n - 1)) {
+      kb = v;
+      ++nkb;
+    }
+  // Lanes with one boundary column and at most one boundary row (all but
+  // grids of a few points) get their boundary values of each level from
+  // registers loaded one step ahead; others load them when needed.
+  const bool slow = nkb > 1 || njb > 1;
 
   // Whole-plane read of level parity p at plane q: boundary planes come
   // from A/B; interior planes from `src` with j/k boundary points patched.
@@ -165,10 +164,44 @@ __device__ __forceinline__ void heat_tile(const GridRound g, int tile,
     }
   };
 
+  // Boundary-column values that step P's level l (plane P-1-l, parity
+  // parity0+l+1) writes into column kb, loaded one step ahead.  (A boundary
+  // row is the same for the whole wave, so only edge waves load it per level.)
+  C bk[T][R], bk_nx[T][R];
+  auto fetch_bnd = [&](C (&ck)[T][R], int P) {
+#pragma unroll
+    for (int l = 0; l < T; ++l) {
+      const int q = P - 1 - l;
+      if (l >= d || q < 1 || q > n - 2 || slow || kb < 0) continue;
+      const S* b = bnd<S>(gA, gB, g.parity0 + l + 1);
+      const int64_t base = static_cast<int64_t>(q) * plane;
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (row_live[r]) ck[l][r] = St::load(b[base + rowoff[r] + kb]);
+    }
+  };
+
   C win[T][3][R][V];  // per level: [0] plane q-1, [1] plane q, [2] plane q+1
+  C pre[R][V];        // level-0 plane P+1, loaded one step ahead
+  const bool kb_lo = kb == 0, kb_hi = kb == V - 1;
   const int pbeg = max(i0 - d, 0), pend = i1 - 1 + d;
+  if (pbeg <= n - 1) read_plane(pre, pbeg, g.parity0, X);
+  fetch_bnd(bk_nx, pbeg);
+  // One step per plane.  (Unrolling by three to rotate the window slots
+  // instead of moving them made the kernel slower: the code no longer fits
+  // the instruction cache.)
   for (int P = pbeg; P <= pend; ++P) {
-    if (P <= n - 1) read_plane(win[0][2], P, g.parity0, X);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int v = 0; v < V; ++v) win[0][2][r][v] = pre[r][v];
+#pragma unroll
+    for (int l = 0; l < T; ++l)
+#pragma unroll
+      for (int r = 0; r < R; ++r) bk[l][r] = bk_nx[l][r];
+    // the next step's loads stay in flight across this step's barriers
+    if (P + 1 <= n - 1 && P + 1 <= pend) read_plane(pre, P + 1, g.parity0, X);
+    if (P + 1 <= pend) fetch_bnd(bk_nx, P + 1);
 #pragma unroll
     for (int l = 0; l < T; ++l) {
       if (l >= d) break;
@@ -205,7 +238,7 @@ __device__ __forceinline__ void heat_tile(const GridRound g, int tile,
                                                         win[l][0][r][v], jp[v], jm[v], kp, km)));
           }
         }
-        if (fixed) {
+        if (slow) {
           const S* b = bnd<S>(gA, gB, g.parity0 + l + 1);
           const int64_t base = static_cast<int64_t>(q) * plane;
 #pragma unroll
@@ -213,6 +246,19 @@ __device__ __forceinline__ void heat_tile(const GridRound g, int tile,
 #pragma unroll
             for (int v = 0; v < V; ++v)
               if (fixed & (1u << (r * V + v))) nw[r][v] = St::load(b[base + rowoff[r] + v]);
+        } else {
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            if (kb_lo) nw[r][0] = bk[l][r];
+            if (kb_hi) nw[r][V - 1] = bk[l][r];
+          }
+          if (jb >= 0) {
+            const S* b = bnd<S>(gA, gB, g.parity0 + l + 1);
+            const int64_t base = static_cast<int64_t>(q) * plane;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+              if (r == jb) load16(b + base + rowoff[r], nw[r]);
+          }
         }
       } else if (q == 0 || q == n - 1) {
         read_plane(nw, q, g.parity0 + l + 1, X);
@@ -283,11 +329,16 @@ __device__ __forceinline__ void jacobi_tile(const GridRound g, int tile,
   bool row_live[R];
   uint64_t fixed = 0;  // bit r*V+v (R*V <= 64)
   uint64_t brow = 0;   // rows that are boundary rows
+  int kb = -1, jb = -1, nkb = 0, njb = 0;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int i = il + w * R + r;
     row_live[r] = vlive && i >= 0 && i < n;
-    if (row_live[r] && (i == 0 || i == n - 1)) brow |= 1ull << r;
+    if (row_live[r] && (i == 0 || i == n - 1)) {
+      brow |= 1ull << r;
+      jb = r;
+      ++njb;
+    }
 #pragma unroll
     for (int v = 0; v < V; ++v) {
       const int k = k0 + v;
@@ -295,7 +346,28 @@ __device__ __forceinline__ void jacobi_tile(const GridRound g, int tile,
         fixed |= 1ull << (r * V + v);
     }
   }
+#pragma unroll
+  for (int v = 0; v < V; ++v)
+    if (vlive && (k0 + v == 0 || k0 + v == n - 1)) {
+      kb = v;
+      ++nkb;
+    }
+  const bool slow = nkb > 1 || njb > 1;
   auto off = [&](int r) { return static_cast<int64_t>(il + w * R + r) * n + k0; };
+  // Boundary column / row values of both parities, loaded once per tile.
+  C kbv[2][R], jbv[2][V];
+  if (!slow) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const S* b = p ? gB : gA;
+      if (kb >= 0) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (row_live[r]) kbv[p][r] = St::load(b[off(r) + kb]);
+      }
+      if (jb >= 0) load16(b + off(jb), jbv[p]);
+    }
+  }
 
   C cur[R][V];
   {
@@ -341,7 +413,13 @@ __device__ __forceinline__ void jacobi_tile(const GridRound g, int tile,
         const C east = v < V - 1 ? cur[r][v + 1] : right;
         // rounded through the storage type, as the plain sweep's store does
         nw[v] = St::load(St::store(jacobi_point<C>(cur[r][v], west, east, south[v], north[v])));
-        if ((fixed >> (r * V + v)) & 1) nw[v] = St::load(b[off(r) + v]);
+        if (slow) {
+          if ((fixed >> (r * V + v)) & 1) nw[v] = St::load(b[off(r) + v]);
+        } else {
+          const int p = (g.parity0 + l + 1) & 1;
+          if (v == kb) nw[v] = p ? kbv[1][r] : kbv[0][r];
+          if (r == jb) nw[v] = p ? jbv[1][v] : jbv[0][v];
+        }
       }
 #pragma unroll
       for (int v = 0; v < V; ++v) {
@@ -386,27 +464,27 @@ __device__ __forceinline__ int round_of(const RoundBatch& b, int t) {
   return lo;
 }
 
-template <class S, int T>
+template <class S, int T, int RH>
 __device__ __forceinline__ void run_tile(const RoundBatch& b, int t,
                                          typename Storage<S>::C* lds) {
   const int g = round_of(b, t);
   const GridRound r = b.round[g];
   if (r.kind == kHeat3D)
-    heat_tile<S, T>(r, t - b.cum[g], lds);
+    heat_tile<S, T, RH>(r, t - b.cum[g], lds);
   else
     jacobi_tile<S, T>(r, t - b.cum[g], lds);
 }
 
 // queue == nullptr: one workgroup per tile (tile = blockIdx.x).  Otherwise a
 // persistent workgroup pulls tile indices from `queue` until they run out.
-template <class S, int T>
+template <class S, int T, int RH>
 __global__ __launch_bounds__(kThreads) void stencil_tiles(const RoundBatch batch,
                                                           QueueState* queue) {
   using C = typename Storage<S>::C;
   __shared__ C lds[TileLds<S, T>::ELEMS];
   const int ntiles = batch.cum[batch.count];
   if (queue == nullptr) {
-    run_tile<S, T>(batch, blockIdx.x, lds);
+    run_tile<S, T, RH>(batch, blockIdx.x, lds);
     return;
   }
   __shared__ int s_tile;
@@ -416,7 +494,7 @@ __global__ __launch_bounds__(kThreads) void stencil_tiles(const RoundBatch batch
     const int t = s_tile;
     __syncthreads();
     if (t >= ntiles) break;
-    run_tile<S, T>(batch, t, lds);
+    run_tile<S, T, RH>(batch, t, lds);
     __syncthreads();
   }
   if (threadIdx.x == 0) {
@@ -429,6 +507,12 @@ __global__ __launch_bounds__(kThreads) void stencil_tiles(const RoundBatch batch
 }
 
 // ------------------------------------------------------------- host planning
+// Tuning knobs (npbx_set_tuning): heat rows per wave and default heat
+// planes per tile.  Read when a launch is planned.
+std::atomic<int> g_heat_rows{4};
+std::atomic<int> g_heat_seg{32};
+std::atomic<int> g_sweep_shape{0};
+
 int max_depth(int dtype) {
   switch (dtype) {
     case NPBX_F64: return MaxDepth<double>::value;
@@ -443,10 +527,11 @@ int vec_elems(int dtype) { return static_cast<int>(16 / dtype_size(dtype)); }
 // Fills the tiling fields of a GridRound for side n and max depth T.
 void tile_grid(GridRound& g, int kind, int n, int dtype, int T, int seg) {
   const int V = vec_elems(dtype);
+  g.rows = g_heat_rows.load();
   const int KT = 64 * V;
   g.tiles_k = n <= KT ? 1 : static_cast<int>(cdiv(n, KT - 2 * V));
   if (kind == kHeat3D) {
-    g.tiles_j = static_cast<int>(cdiv(n - 2, kWaves * kRowsHeat - 2 * T));
+    g.tiles_j = static_cast<int>(cdiv(n - 2, kWaves * g.rows - 2 * T));
     g.seg = seg;
   } else {
     g.tiles_j = static_cast<int>(cdiv(n - 2, kWaves * jacobi_rows(V) - 2 * T));
@@ -476,61 +561,71 @@ std::vector<int> round_depths(int64_t nsweeps, int T) {
   return out;
 }
 
-template <class S, int T>
+template <class S, int T, int RH>
 int launch_tiles(const RoundBatch& b, QueueState* queue, int nblocks, hipStream_t stream) {
-  hipLaunchKernelGGL((stencil_tiles<S, T>), dim3(static_cast<unsigned>(nblocks)), dim3(kThreads),
-                     0, stream, b, queue);
+  hipLaunchKernelGGL((stencil_tiles<S, T, RH>), dim3(static_cast<unsigned>(nblocks)),
+                     dim3(kThreads), 0, stream, b, queue);
   NPBX_LAUNCHED("stencil_tiles");
   return NPBX_OK;
 }
 
-template <class S>
-int dispatch_T(int T, const RoundBatch& b, QueueState* queue, int nblocks, hipStream_t stream) {
-  switch (T) {
-    case 2: return launch_tiles<S, 2>(b, queue, nblocks, stream);
-    case 3:
-      if constexpr (MaxDepth<S>::value >= 3) return launch_tiles<S, 3>(b, queue, nblocks, stream);
-      break;
-    case 4:
-      if constexpr (MaxDepth<S>::value >= 4) return launch_tiles<S, 4>(b, queue, nblocks, stream);
-      break;
-    default: break;
+// Calls f.template operator()<T, RH>() for an instantiated (T, RH) pair.
+template <class S, class F>
+int with_shape(int T, int RH, F&& f) {
+  if constexpr (MaxDepth<S>::value >= 2) {
+    if (T == 2 && RH == 2) return f.template operator()<2, 2>();
+    if (T == 2 && RH == 4) return f.template operator()<2, 4>();
   }
-  return fail(NPBX_EUNSUPPORTED, "tile depth T=%d not supported for this dtype", T);
+  if constexpr (sizeof(S) >= 4) {  // BF16 windows with 6 or 8 rows spill
+    if (T == 2 && RH == 6) return f.template operator()<2, 6>();
+  }
+  if constexpr (sizeof(S) == 4) {  // and FP64 ones with 8
+    if (T == 2 && RH == 8) return f.template operator()<2, 8>();
+  }
+  if constexpr (MaxDepth<S>::value >= 3) {
+    if (T == 3 && RH == 4) return f.template operator()<3, 4>();
+    if (T == 3 && RH == 6) return f.template operator()<3, 6>();
+  }
+  if constexpr (MaxDepth<S>::value >= 4) {
+    if (T == 4 && RH == 4) return f.template operator()<4, 4>();
+  }
+  return fail(NPBX_EUNSUPPORTED, "tile depth T=%d with %d heat rows per wave is not built for "
+              "this dtype", T, RH);
 }
 
-int launch_any(int dtype, int T, const RoundBatch& b, QueueState* queue, int nblocks,
+int launch_any(int dtype, int T, int RH, const RoundBatch& b, QueueState* queue, int nblocks,
                hipStream_t stream) {
   return dispatch_dtype(dtype, [&](auto tag) {
     using S = decltype(tag);
-    return dispatch_T<S>(T, b, queue, nblocks, stream);
+    return with_shape<S>(T, RH, [&]<int TT, int RR>() {
+      return launch_tiles<S, TT, RR>(b, queue, nblocks, stream);
+    });
   });
 }
 
 // Workgroups that fit on the device at once for the persistent grid.
-int resident_blocks(int dtype, int T) {
+int resident_blocks(int dtype, int T, int RH) {
   int dev = 0, cus = 0, per_cu = 1;
   if (hipGetDevice(&dev) != hipSuccess) return 1024;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     cus = 256;
-  auto occ = [&](auto kern) {
-    int b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kern, kThreads, 0) == hipSuccess && b > 0)
-      per_cu = b;
-  };
   dispatch_dtype(dtype, [&](auto tag) {
     using S = decltype(tag);
-    if (T == 2) occ(stencil_tiles<S, 2>);
-    if constexpr (MaxDepth<S>::value >= 4) {
-      if (T == 3) occ(stencil_tiles<S, 3>);
-      if (T == 4) occ(stencil_tiles<S, 4>);
-    }
-    return NPBX_OK;
+    return with_shape<S>(T, RH, [&]<int TT, int RR>() {
+      int b = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, stencil_tiles<S, TT, RR>, kThreads,
+                                                       0) == hipSuccess &&
+          b > 0)
+        per_cu = b;
+      return NPBX_OK;
+    });
   });
   return cus * per_cu;
 }
 
 }  // namespace
+
+int sweep_shape() { return g_sweep_shape.load(); }
 
 // Single grid, temporal-blocked: rounds of depth <= T through the tile
 // kernel (one workgroup per tile), then the final depth-1 round.
@@ -546,7 +641,7 @@ int tiled_sweeps(int kind, int dtype, void* A, void* B, int n, int64_t first, in
   g.B = B;
   g.n = n;
   g.kind = kind;
-  tile_grid(g, kind, n, dtype, T, seg > 0 ? seg : 32);
+  tile_grid(g, kind, n, dtype, T, seg > 0 ? seg : g_heat_seg.load());
   const int64_t ntiles = tiles_per_round(g);
   NPBX_REQUIRE(ntiles < (int64_t(1) << 31), "too many tiles");
   int64_t level = first;
@@ -563,7 +658,7 @@ int tiled_sweeps(int kind, int dtype, void* A, void* B, int n, int64_t first, in
     g.parity0 = static_cast<int>(level & 1);
     g.depth = depth;
     b.round[0] = g;
-    NPBX_TRY(launch_any(dtype, T, b, nullptr, static_cast<int>(ntiles), stream));
+    NPBX_TRY(launch_any(dtype, T, g.rows, b, nullptr, static_cast<int>(ntiles), stream));
     X = g.Y;
     level += depth;
   }
@@ -625,6 +720,7 @@ int64_t grid_lups(const GridRound& g, int64_t sweeps) {
 struct Plan {
   int dtype = 0;
   int T = 0;
+  int rows = 4;
   int nblocks = 0;
   std::vector<RoundBatch> launches;
   QueueState* d_queue = nullptr;
@@ -647,7 +743,7 @@ int plan_enqueue(Plan* p, hipStream_t s) {
   for (const RoundBatch& b : p->launches) {
     const int nt = b.cum[b.count];
     if (nt == 0) continue;
-    NPBX_TRY(launch_any(p->dtype, p->T, b, p->d_queue, std::min(nt, p->nblocks), s));
+    NPBX_TRY(launch_any(p->dtype, p->T, p->rows, b, p->d_queue, std::min(nt, p->nblocks), s));
   }
   return NPBX_OK;
 }
@@ -670,7 +766,7 @@ extern "C" int npbx_plan_create(const npbx_grid_spec* specs, int32_t count, int3
   const int tmax = max_depth(dtype);
   NPBX_REQUIRE(tblock >= 2 && tblock <= tmax, "plan_create: tblock=%d outside [2, %d] for %s",
                tblock, tmax, dtype_name(dtype));
-  if (seg <= 0) seg = 32;
+  if (seg <= 0) seg = g_heat_seg.load();
   std::vector<GridRound> base(count);
   std::vector<std::vector<int>> depths(count);
   size_t nrounds = 0;
@@ -711,7 +807,8 @@ extern "C" int npbx_plan_create(const npbx_grid_spec* specs, int32_t count, int3
     for (auto& b : pk.out) p->launches.push_back(b);
     p->ntiles += pk.tiles;
   }
-  p->nblocks = resident_blocks(dtype, tblock);
+  p->rows = base[0].rows;
+  p->nblocks = resident_blocks(dtype, tblock, p->rows);
   const int rc = new_queue(&p->d_queue);
   if (rc != NPBX_OK) {
     plan_free(p);
@@ -797,7 +894,7 @@ extern "C" int npbx_rounds_launch(const npbx_round_spec* specs, int32_t count, i
   const int tmax = max_depth(dtype);
   NPBX_REQUIRE(tblock >= 2 && tblock <= tmax, "rounds_launch: tblock=%d outside [2, %d] for %s",
                tblock, tmax, dtype_name(dtype));
-  if (seg <= 0) seg = 32;
+  if (seg <= 0) seg = g_heat_seg.load();
   Packer pk;
   for (int i = 0; i < count; ++i) {
     const npbx_round_spec& s = specs[i];
@@ -814,11 +911,52 @@ extern "C" int npbx_rounds_launch(const npbx_round_spec* specs, int32_t count, i
     g.depth = s.depth;
     pk.add(g);
   }
-  const int nblocks = resident_blocks(dtype, tblock);
+  const int rows = g_heat_rows.load();
+  const int nblocks = resident_blocks(dtype, tblock, rows);
   for (const RoundBatch& b : pk.out) {
     const int nt = b.cum[b.count];
-    NPBX_TRY(launch_any(dtype, tblock, b, static_cast<QueueState*>(queue), std::min(nt, nblocks),
-                        as_stream(stream)));
+    NPBX_TRY(launch_any(dtype, tblock, rows, b, static_cast<QueueState*>(queue),
+                        std::min(nt, nblocks), as_stream(stream)));
   }
   return NPBX_OK;
+}
+
+// ---- tuning ------------------------------------------------------------------
+extern "C" int npbx_set_tuning(const char* key, int64_t value) {
+  NPBX_REQUIRE(key, "set_tuning: null key");
+  if (std::strcmp(key, "heat.rows") == 0) {
+    NPBX_REQUIRE(value == 2 || value == 4 || value == 6 || value == 8,
+                 "set_tuning: heat.rows must be 2, 4, 6 or 8");
+    g_heat_rows.store(static_cast<int>(value));
+    return NPBX_OK;
+  }
+  if (std::strcmp(key, "heat.seg") == 0) {
+    NPBX_REQUIRE(value >= 1 && value <= 4096, "set_tuning: heat.seg must be in [1, 4096]");
+    g_heat_seg.store(static_cast<int>(value));
+    return NPBX_OK;
+  }
+  if (std::strcmp(key, "sweep.shape") == 0) {
+    NPBX_REQUIRE(value >= 0 && value <= 3, "set_tuning: sweep.shape must be in [0, 3]");
+    g_sweep_shape.store(static_cast<int>(value));
+    return NPBX_OK;
+  }
+  return fail(NPBX_EINVAL, "set_tuning: unknown key '%s' (heat.rows, heat.seg, sweep.shape)",
+              key);
+}
+
+extern "C" int npbx_get_tuning(const char* key, int64_t* value) {
+  NPBX_REQUIRE(key && value, "get_tuning: null argument");
+  if (std::strcmp(key, "heat.rows") == 0) {
+    *value = g_heat_rows.load();
+    return NPBX_OK;
+  }
+  if (std::strcmp(key, "heat.seg") == 0) {
+    *value = g_heat_seg.load();
+    return NPBX_OK;
+  }
+  if (std::strcmp(key, "sweep.shape") == 0) {
+    *value = g_sweep_shape.load();
+    return NPBX_OK;
+  }
+  return fail(NPBX_EINVAL, "get_tuning: unknown key '%s'", key);
 }

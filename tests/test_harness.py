@@ -225,3 +225,30 @@ def test_exit_after_interrupts_main_thread():
         return 7
 
     assert quick() == 7
+
+
+# ------------------------------------------------------------------ autotuner
+def test_tuner_search_space_and_cache(tmp_path, monkeypatch):
+    torch = pytest.importorskip("torch")
+    from npbench_amd import tune
+    heat = tune.candidates("heat_3d", torch.float32)
+    assert {"T": 1, "rows": 0, "seg": 0, "shape": 3} in heat
+    assert {(c["T"], c["rows"]) for c in heat if c["T"] > 1} == set(tune.SHAPES[torch.float32])
+    jac = tune.candidates("jacobi_2d", torch.float64)
+    assert [c["T"] for c in jac] == [1, 2]
+    # the cache round-trips without a device (measurement monkeypatched)
+    calls = []
+
+    def fake_measure(kind, dtype, n, count, cfg, sweeps=16, reps=2, seed=0):
+        calls.append(cfg)
+        return 100.0 + cfg["T"] * 10 + cfg["rows"] - cfg["seg"] / 100
+
+    monkeypatch.setattr(tune, "measure", fake_measure)
+    monkeypatch.setattr(tune, "_memo", {})
+    path = tmp_path / "tune.json"
+    cfg = tune.best("heat_3d", torch.float32, 256, 16, cache=path)
+    assert cfg["T"] == 4 and cfg["rows"] == 4 and cfg["seg"] == 32
+    n_calls = len(calls)
+    monkeypatch.setattr(tune, "_memo", {})
+    assert tune.best("heat_3d", torch.float32, 256, 16, cache=path) == cfg
+    assert len(calls) == n_calls  # served from the file
