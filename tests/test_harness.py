@@ -252,3 +252,29 @@ def test_tuner_search_space_and_cache(tmp_path, monkeypatch):
     monkeypatch.setattr(tune, "_memo", {})
     assert tune.best("heat_3d", torch.float32, 256, 16, cache=path) == cfg
     assert len(calls) == n_calls  # served from the file
+
+
+# ------------------------------------------------------------------ report
+def test_speedup_report_from_the_result_store(tmp_path):
+    from npbench_amd import report
+    db = tmp_path / "npbench.db"
+    conn = util.open_results(str(db))
+    rows = []
+    for fw, det, t, ok in (("numpy", "default", 1.0, True), ("mi355x", "default", 0.01, True),
+                           ("mi355x", "bf16", 0.005, False)):
+        for rep in range(5):
+            rows.append({"timestamp": 0, "benchmark": "heat3d", "kind": "", "domain": "",
+                         "dwarf": "", "preset": "S", "mode": "main", "framework": fw,
+                         "version": "1", "details": det, "validated": ok,
+                         "time": t * (1 + 0.01 * rep)})
+    util.insert_results(conn, rows)
+    conn.close()
+    sp = report.speedups(report.load(str(db)))
+    r = sp[("heat3d", "S")]["mi355x"]
+    assert r["details"] == "bf16" and not r["validated"]  # fastest implementation kept
+    assert 150 < r["speedup"] < 250 and r["ci"][0] <= r["speedup"] <= r["ci"][1]
+    md = report.markdown(sp)
+    assert "not validated" in md and "heat3d" in md
+    pytest.importorskip("matplotlib")
+    report.heatmap(sp, str(tmp_path / "s.png"))
+    assert (tmp_path / "s.png").stat().st_size > 0
