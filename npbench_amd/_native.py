@@ -143,6 +143,18 @@ def call(name, *args):
     return rc
 
 
+def get_tuning(key: str) -> int:
+    """Current value of a launch-shape knob (npbx_get_tuning)."""
+    out = _i64(0)
+    call("npbx_get_tuning", key.encode(), ctypes.byref(out))
+    return int(out.value)
+
+
+def set_tuning(key: str, value: int) -> None:
+    """Sets a launch-shape knob (npbx_set_tuning); results never depend on it."""
+    call("npbx_set_tuning", key.encode(), int(value))
+
+
 def version() -> str:
     buf = ctypes.create_string_buffer(64)
     call("npbx_version", buf, len(buf))

@@ -11,7 +11,19 @@ namespace npbx {
 // summed i, j, k, then + c -- so it is bitwise NumPy.  FP32 arithmetic (FP32
 // and BF16 storage) evaluates the same affine map as
 // 0.125*(sum of the six neighbours) + 0.25*c with one FMA: 7 operations
-// instead of 13, within an ulp or two of the reference order per sweep.
+// instead of 13, within an ulp or two of the reference order per sweep.  The
+// six neighbours are summed plane i-1 first and plane i+1 last, so a kernel
+// marching along i can form the first five-term partial sum as soon as plane
+// i arrives (heat_partial) and finish the point when plane i+1 does
+// (heat_finish); every FP32 kernel evaluates exactly this sequence.
+__device__ __forceinline__ float heat_partial(float im, float jm, float jp, float km, float kp) {
+  return (((im + jm) + jp) + km) + kp;
+}
+
+__device__ __forceinline__ float heat_finish(float partial, float ip, float c) {
+  return __builtin_fmaf(0.125f, partial + ip, 0.25f * c);
+}
+
 template <class C>
 __device__ __forceinline__ C heat_point(C c, C ip, C im, C jp, C jm, C kp, C km) {
   const C c2 = static_cast<C>(2.0) * c;
@@ -25,8 +37,7 @@ __device__ __forceinline__ C heat_point(C c, C ip, C im, C jp, C jm, C kp, C km)
 template <>
 __device__ __forceinline__ float heat_point<float>(float c, float ip, float im, float jp,
                                                    float jm, float kp, float km) {
-  const float s = ((((ip + im) + jp) + jm) + kp) + km;
-  return __builtin_fmaf(0.125f, s, 0.25f * c);
+  return heat_finish(heat_partial(im, jm, jp, km, kp), ip, c);
 }
 
 // Jacobi (jacobi_2d_numpy.py:7-10): ((((c + W) + E) + S) + N) * 0.2, already

@@ -37,6 +37,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -300,6 +301,299 @@ __device__ __forceinline__ void heat_tile(const GridRound g, int tile,
   }
 }
 
+// ------------------------------------------------- heat 3-D, wave-private tile
+// FP32 storage only.  Each wave owns JR rows (j) x 256 columns (k) and needs
+// nothing from the other waves: its rows include a halo of T rows on each
+// side, so the tile uses no LDS and no barrier.  A workgroup's four waves
+// cover 4*(JR-2T) consecutive output rows; the halo rows two waves share are
+// read by both (an L2 hit for the second).
+//
+// Marching along i, level l keeps per row a[l], its values at its newest
+// plane q, and part[l], the five-term partial sums (heat_partial) of level
+// l+1 at plane q.  When level l's plane q+1 arrives (from HBM for level 0,
+// computed for l >= 1) level l+1 at plane q is finished (heat_finish) and
+// arrives at level l+1.  Level l's rows shrink by one per side per level, so
+// the wave holds JR, JR-2, ... rows.  k neighbours across lanes come from
+// full-wave DPP shifts.  Level 0's plane P+1 is loaded while levels >= 1 of
+// step P are computed.
+//
+// Global-boundary points of level L hold A's (L even) or B's (L odd) values.
+// Level 0's boundary rows and planes are read from that array instead of X;
+// its boundary columns, and those of intermediate levels, are patched from a
+// lane-spread register (lane r < JR: row r's column 0; lane 32 + r: its
+// column n-1) loaded one step ahead; an intermediate level's boundary row is
+// loaded one step ahead too (synchronously in grids of a few rows).
+constexpr int kPrivateRows = 10;  // heat.rows >= this selects the wave-private tile
+constexpr bool wave_private(int rows) { return rows >= kPrivateRows; }
+
+// Raw buffer access to one plane (num_records = the plane's bytes; the
+// descriptor's third word is the gfx9 raw-buffer format).  The 32-bit vector
+// offset is the lane's, the scalar offset the wave's row.
+struct Rsrc {
+  __amdgpu_buffer_rsrc_t r;
+};
+__device__ __forceinline__ Rsrc plane_rsrc(const float* base, int64_t elems) {
+  return {__builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), static_cast<short>(0),
+                                            static_cast<int>(elems * 4), 0x00020000)};
+}
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void buf_load16(Rsrc rs, uint32_t voff, uint32_t soff, float (&d)[4]) {
+  const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rs.r, voff, soff, 0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) d[i] = __uint_as_float(v[i]);
+}
+__device__ __forceinline__ float buf_load1(Rsrc rs, uint32_t voff, uint32_t soff) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs.r, voff, soff, 0));
+}
+__device__ __forceinline__ void buf_store16(Rsrc rs, uint32_t voff, uint32_t soff,
+                                            const float (&d)[4]) {
+  u32x4 v;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v[i] = __float_as_uint(d[i]);
+  __builtin_amdgcn_raw_buffer_store_b128(v, rs.r, voff, soff, 0);
+}
+__device__ __forceinline__ void buf_store1(Rsrc rs, uint32_t voff, uint32_t soff, float x) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), rs.r, voff, soff, 0);
+}
+
+// Full-wave DPP shifts (wave_shr:1, wave_shl:1): lane l receives lane l-1's
+// (from_left) or lane l+1's (from_right) value; lane 0 / lane 63 receive 0,
+// which only ever reaches halo or global-boundary points.
+__device__ __forceinline__ float from_left(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float from_right(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float lane_value(float x, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), lane));
+}
+
+// One tile at compile-time depth D (1 <= D <= T).  Every step computes every
+// level unconditionally: planes a level produces before its inputs are valid
+// (while the march fills) or past the grid (loads clamp to the last plane)
+// only ever feed other such planes or boundary planes, and only level D's
+// planes inside the output segment are stored.
+//
+// The loop's memory accesses are unconditional, so the compiler waits for
+// exactly the loads a step consumes.  Stores that must not happen (rows past
+// the wave's output rows, planes outside the segment, lanes outside the
+// output columns) go through an empty descriptor or an out-of-range offset,
+// which the hardware drops; a stored vector that holds a global-boundary
+// column carries that column's current value of Y, rewritten unchanged.
+// Level-0 planes and all patch values are loaded two steps ahead into two
+// register sets that the twice-unrolled loop uses in turn.
+template <int T, int JR, int D>
+__device__ __forceinline__ void heat_wave_tile(const GridRound g, int tile) {
+  constexpr int V = 4;
+  constexpr int KT = 64 * V;
+  constexpr int WJ = JR - 2 * T;  // output rows of a wave
+  static_assert(WJ >= 1 && JR <= 32, "a wave holds 2T halo rows plus its output rows, <= 32");
+  static_assert(D >= 1 && D <= T, "depth");
+  constexpr uint32_t kDrop = 0x80000000u;  // beyond every plane descriptor's range
+  const int n = g.n;
+  // the wave index, known to the compiler as wave-uniform: row bases and row
+  // conditions then live in scalar registers and branches
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int tk = tile % g.tiles_k;
+  const int rest = tile / g.tiles_k;
+  const int tj = rest % g.tiles_j;
+  const int ti = rest / g.tiles_j;
+  const int jl = 1 + (tj * kWaves + w) * WJ - T;  // row of register row 0
+  const int jlo = jl + T, jhi = min(jl + T + WJ, n - 1);
+  if (jlo >= jhi) return;  // the wave lies past the last interior row
+  int kl, klo, khi;
+  col_range<KT, V>(n, g.tiles_k, tk, kl, klo, khi);
+  const int i0 = 1 + ti * g.seg, i1 = min(i0 + g.seg, n - 1);
+  const int k0 = kl + lane * V;
+  const bool vlive = k0 >= 0 && k0 + V <= n;
+  // Accesses go through buffer descriptors of one plane: a wave-uniform row
+  // offset (scalar) plus the lane's 32-bit byte offset within the row (lanes
+  // outside the grid read the row's first vector).  Planes and rows are
+  // clamped into the grid.  Lanes whose vector holds output points store it.
+  const uint32_t lob = vlive ? static_cast<uint32_t>(k0) * 4u : 0u;
+  const uint32_t sob = (vlive && k0 + V > klo && k0 < khi) ? lob : kDrop;
+  const int64_t plane = static_cast<int64_t>(n) * n;
+  const float* X = static_cast<const float*>(g.X);
+  float* Y = static_cast<float*>(g.Y);
+  const float* gA = static_cast<const float*>(g.A);
+  const float* gB = static_cast<const float*>(g.B);
+  const int p0 = g.parity0;
+  auto bsrc = [&](int level) { return ((p0 + level) & 1) ? gB : gA; };
+  auto row_bnd = [&](int r) { return jl + r == 0 || jl + r == n - 1; };
+  auto plane_of = [&](const float* base, int q) {
+    return plane_rsrc(base + static_cast<int64_t>(min(max(q, 0), n - 1)) * plane, plane);
+  };
+  auto rowoff = [&](int r) { return static_cast<uint32_t>(min(max(jl + r, 0), n - 1) * n) * 4u; };
+  auto ld = [&](Rsrc rs, int r, float (&dst)[V]) { buf_load16(rs, lob, rowoff(r), dst); };
+
+  // Boundary row among the rows intermediate levels compute (1 .. JR-2).
+  int jb = -1, njb = 0;
+#pragma unroll
+  for (int r = 1; r < JR - 1; ++r)
+    if (row_bnd(r)) {
+      jb = r;
+      ++njb;
+    }
+  const bool slow = njb > 1;  // grids of a few rows: load boundary rows when needed
+  const int jbr = jb >= 0 ? jb : 0;  // the row the boundary-row prefetch reads
+
+  // Boundary columns: lane-spread values (lane r < JR: row r's column 0;
+  // lane 32 + r: its column n-1), patched into the lanes that hold them.
+  const int kr = lane & 31;
+  const int kcol = lane >= 32 ? n - 1 : 0;
+  const bool is_k0 = vlive && k0 == 0, is_kn = vlive && k0 + V == n;
+  const uint32_t kboff =
+      (static_cast<uint32_t>(min(max(jl + kr, 0), n - 1) * n) + static_cast<uint32_t>(kcol)) * 4u;
+  auto load_cols = [&](const float* base, int q) { return buf_load1(plane_of(base, q), kboff, 0u); };
+  auto patch_k = [&](float (&x)[JR][V], float kb, int rlo, int rhi) {
+#pragma unroll
+    for (int r = 0; r < JR; ++r) {
+      if (r < rlo || r >= rhi) continue;
+      const float vlo = lane_value(kb, r), vhi = lane_value(kb, 32 + r);
+      if (is_k0) x[r][0] = vlo;
+      if (is_kn) x[r][V - 1] = vhi;
+    }
+  };
+
+  // What step S consumes: level-0 plane S and its boundary columns; level
+  // m's boundary columns and row at plane S - m (1 <= m < D); Y's boundary
+  // columns at the stored plane S - D.
+  struct Ahead {
+    float pre[JR][V];
+    float kq[T];
+    float jrow[T][V];
+    float ky;
+  };
+  auto fetch = [&](int S, Ahead& f) {
+    const float* b = bsrc(0);
+    const bool bplane = S <= 0 || S >= n - 1;
+    const Rsrc rx = plane_of(bplane ? b : X, S), rb = plane_of(b, S);
+#pragma unroll
+    for (int r = 0; r < JR; ++r) ld(row_bnd(r) ? rb : rx, r, f.pre[r]);
+    f.kq[0] = load_cols(b, S);
+#pragma unroll
+    for (int m = 1; m < D; ++m) {
+      f.kq[m] = load_cols(bsrc(m), S - m);
+      ld(plane_of(bsrc(m), S - m), jbr, f.jrow[m]);
+    }
+    f.ky = load_cols(Y, S - D);
+  };
+
+  float a[T][JR][V];  // level l at its newest plane (rows l+1 .. JR-l-2)
+  float part[T][JR][V];
+#pragma unroll
+  for (int l = 0; l < T; ++l)
+#pragma unroll
+    for (int r = 0; r < JR; ++r)
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        a[l][r][v] = 0.0f;
+        part[l][r][v] = 0.0f;
+      }
+
+  // One step: consumes f (loaded two steps ago), then refills it for step P+2.
+  auto step = [&](int P, Ahead& f) {
+    float in[JR][V];  // the arriving plane of the current level
+#pragma unroll
+    for (int r = 0; r < JR; ++r)
+#pragma unroll
+      for (int v = 0; v < V; ++v) in[r][v] = f.pre[r][v];
+    patch_k(in, f.kq[0], 0, JR);
+    float kq[T], jrow[T][V], ky = f.ky;
+#pragma unroll
+    for (int m = 0; m < T; ++m) {
+      kq[m] = f.kq[m];
+#pragma unroll
+      for (int v = 0; v < V; ++v) jrow[m][v] = f.jrow[m][v];
+    }
+    fetch(P + 2, f);
+#pragma unroll
+    for (int l = 0; l < D; ++l) {
+      const int m = l + 1;  // the level produced, at plane q
+      const int q = P - m;
+      float o[JR][V];
+#pragma unroll
+      for (int r = m; r < JR - m; ++r)
+#pragma unroll
+        for (int v = 0; v < V; ++v) o[r][v] = heat_finish(part[l][r][v], in[r][v], a[l][r][v]);
+      if (m == D) {
+        patch_k(o, ky, T, JR - T);
+        const bool inseg = q >= i0 && q < i1;
+        const Rsrc ry = plane_rsrc(Y + static_cast<int64_t>(inseg ? q : i0) * plane, plane);
+        const Rsrc none = plane_rsrc(Y, 0);
+#pragma unroll
+        for (int r = T; r < JR - T; ++r)
+          buf_store16((inseg && jl + r < jhi) ? ry : none, sob, rowoff(r), o[r]);
+      } else if (q == 0 || q == n - 1) {
+        // a boundary plane of an intermediate level (first and last tiles
+        // along i).  The wait for every load on this rare path keeps the
+        // compiler from waiting for these loads on the common one.
+        const Rsrc rb = plane_of(bsrc(m), q);
+#pragma unroll
+        for (int r = m; r < JR - m; ++r) ld(rb, r, o[r]);
+        __builtin_amdgcn_s_waitcnt(0);
+      } else {
+        patch_k(o, kq[m < T ? m : T - 1], m, JR - m);
+        if (slow) {
+          const Rsrc rb = plane_of(bsrc(m), q);
+#pragma unroll
+          for (int r = m; r < JR - m; ++r)
+            if (row_bnd(r)) ld(rb, r, o[r]);
+          __builtin_amdgcn_s_waitcnt(0);
+        } else {
+#pragma unroll
+          for (int r = m; r < JR - m; ++r)
+            if (r == jb)
+#pragma unroll
+              for (int v = 0; v < V; ++v) o[r][v] = jrow[m < T ? m : T - 1][v];
+        }
+      }
+      // level l's arrival (plane P - l) becomes its newest plane
+#pragma unroll
+      for (int r = m; r < JR - m; ++r) {
+        const float left = from_left(in[r][V - 1]);
+        const float right = from_right(in[r][0]);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          const float km = v > 0 ? in[r][v - 1] : left;
+          const float kp = v < V - 1 ? in[r][v + 1] : right;
+          part[l][r][v] = heat_partial(a[l][r][v], in[r - 1][v], in[r + 1][v], km, kp);
+          a[l][r][v] = in[r][v];
+        }
+      }
+      if (m < D) {
+#pragma unroll
+        for (int r = m; r < JR - m; ++r)
+#pragma unroll
+          for (int v = 0; v < V; ++v) in[r][v] = o[r][v];
+      }
+    }
+  };
+
+  const int pbeg = max(i0 - D, 0), pfin = i1 - 1 + D;
+  Ahead fa, fb;
+  fetch(pbeg, fa);
+  fetch(pbeg + 1, fb);
+  for (int P = pbeg; P <= pfin; P += 2) {
+    step(P, fa);
+    if (P + 1 > pfin) break;
+    step(P + 1, fb);
+  }
+}
+
+// Dispatches a round's runtime depth to the compile-time one.
+template <int T, int JR>
+__device__ __forceinline__ void heat_wave_round(const GridRound g, int tile) {
+  if constexpr (T >= 2) {
+    if (g.depth == 2) return heat_wave_tile<T, JR, 2>(g, tile);
+  }
+  if constexpr (T >= 3) {
+    if (g.depth == 3) return heat_wave_tile<T, JR, 3>(g, tile);
+  }
+  heat_wave_tile<T, JR, 1>(g, tile);
+}
+
 // ---------------------------------------------------------------- jacobi 2-D
 template <class S, int T>
 __device__ __forceinline__ void jacobi_tile(const GridRound g, int tile,
@@ -444,12 +738,12 @@ __device__ __forceinline__ void jacobi_tile(const GridRound g, int tile,
   }
 }
 
-template <class S, int T>
+template <class S, int T, int RH>
 struct TileLds {
   using C = typename Storage<S>::C;
   static constexpr int V = Vec16<S>::N;
   static constexpr int KT = 64 * V;
-  static constexpr int HEAT = T * kWaves * 2 * KT;
+  static constexpr int HEAT = wave_private(RH) ? 0 : T * kWaves * 2 * KT;
   static constexpr int JACOBI = kWaves * 2 * KT;  // one exchange buffer
   static constexpr int ELEMS = HEAT > JACOBI ? HEAT : JACOBI;
 };
@@ -469,10 +763,14 @@ __device__ __forceinline__ void run_tile(const RoundBatch& b, int t,
                                          typename Storage<S>::C* lds) {
   const int g = round_of(b, t);
   const GridRound r = b.round[g];
-  if (r.kind == kHeat3D)
-    heat_tile<S, T, RH>(r, t - b.cum[g], lds);
-  else
+  if (r.kind == kHeat3D) {
+    if constexpr (wave_private(RH))
+      heat_wave_round<T, RH>(r, t - b.cum[g]);
+    else
+      heat_tile<S, T, RH>(r, t - b.cum[g], lds);
+  } else {
     jacobi_tile<S, T>(r, t - b.cum[g], lds);
+  }
 }
 
 // queue == nullptr: one workgroup per tile (tile = blockIdx.x).  Otherwise a
@@ -481,7 +779,7 @@ template <class S, int T, int RH>
 __global__ __launch_bounds__(kThreads) void stencil_tiles(const RoundBatch batch,
                                                           QueueState* queue) {
   using C = typename Storage<S>::C;
-  __shared__ C lds[TileLds<S, T>::ELEMS];
+  __shared__ C lds[TileLds<S, T, RH>::ELEMS];
   const int ntiles = batch.cum[batch.count];
   if (queue == nullptr) {
     run_tile<S, T, RH>(batch, blockIdx.x, lds);
@@ -531,7 +829,10 @@ void tile_grid(GridRound& g, int kind, int n, int dtype, int T, int seg) {
   const int KT = 64 * V;
   g.tiles_k = n <= KT ? 1 : static_cast<int>(cdiv(n, KT - 2 * V));
   if (kind == kHeat3D) {
-    g.tiles_j = static_cast<int>(cdiv(n - 2, kWaves * g.rows - 2 * T));
+    // a shared tile is 4*rows rows with one halo; a wave-private one is four
+    // waves of rows-2T output rows each
+    const int out_rows = wave_private(g.rows) ? kWaves * (g.rows - 2 * T) : kWaves * g.rows - 2 * T;
+    g.tiles_j = static_cast<int>(cdiv(n - 2, out_rows));
     g.seg = seg;
   } else {
     g.tiles_j = static_cast<int>(cdiv(n - 2, kWaves * jacobi_rows(V) - 2 * T));
@@ -588,6 +889,10 @@ int with_shape(int T, int RH, F&& f) {
   }
   if constexpr (MaxDepth<S>::value >= 4) {
     if (T == 4 && RH == 4) return f.template operator()<4, 4>();
+  }
+  if constexpr (std::is_same_v<S, float>) {  // wave-private heat tiles
+    if (T == 2 && RH == 10) return f.template operator()<2, 10>();
+    if (T == 2 && RH == 12) return f.template operator()<2, 12>();
   }
   return fail(NPBX_EUNSUPPORTED, "tile depth T=%d with %d heat rows per wave is not built for "
               "this dtype", T, RH);
@@ -925,8 +1230,10 @@ extern "C" int npbx_rounds_launch(const npbx_round_spec* specs, int32_t count, i
 extern "C" int npbx_set_tuning(const char* key, int64_t value) {
   NPBX_REQUIRE(key, "set_tuning: null key");
   if (std::strcmp(key, "heat.rows") == 0) {
-    NPBX_REQUIRE(value == 2 || value == 4 || value == 6 || value == 8,
-                 "set_tuning: heat.rows must be 2, 4, 6 or 8");
+    NPBX_REQUIRE(value == 2 || value == 4 || value == 6 || value == 8 || value == 10 ||
+                     value == 12,
+                 "set_tuning: heat.rows must be 2, 4, 6, 8 (shared tile) or 10, 12 "
+                 "(wave-private FP32 tile)");
     g_heat_rows.store(static_cast<int>(value));
     return NPBX_OK;
   }

@@ -87,7 +87,10 @@ __global__ __launch_bounds__(kWave) void vadv_wave(S* __restrict__ utens_stage,
   __shared__ double base_lds[KC * kPad];
   __shared__ double u_lds[KC * kPad];
   const int lane = threadIdx.x;
-  const int64_t c0 = static_cast<int64_t>(blockIdx.x) * kWave;
+  // XCD-contiguous numbering: the waves of columns (i, j) and (i+1, j), which
+  // both read wcon's row i+1, run on one XCD at about the same time and share
+  // its L2.
+  const int64_t c0 = static_cast<int64_t>(xcd_tile(blockIdx.x, gridDim.x)) * kWave;
   const int ncol = static_cast<int>(std::min<int64_t>(kWave, ncols - c0));
   const bool active = lane < ncol;
   const int64_t col = c0 + lane;

@@ -170,3 +170,52 @@ def test_plan_rejects_bad_grids(st):
     a = device.to_device(np.zeros((12, 12)), torch.float64)
     with pytest.raises(_native.NpbxError):
         batch.Plan([batch.Grid("jacobi_2d", a, a.clone(), 4)], tblock=3)  # FP64 max T is 2
+
+
+@pytest.fixture
+def heat_rows():
+    """Sets the library's heat.rows knob for one test and restores it."""
+    from npbench_amd import _native
+    old = _native.get_tuning("heat.rows")
+
+    yield lambda rows: _native.set_tuning("heat.rows", rows)
+    _native.set_tuning("heat.rows", old)
+
+
+WAVE_HEAT_CASES = [4, 8, 12, 20, 68, 264, 520]
+
+
+@pytest.mark.parametrize("rows", [10, 12])
+@pytest.mark.parametrize("n", WAVE_HEAT_CASES)
+@pytest.mark.parametrize("nsweeps,first", [(1, 1), (2, 0), (3, 0), (6, 1), (11, 0)])
+def test_heat_wave_tiles_match_plain(st, heat_rows, rows, n, nsweeps, first):
+    """The wave-private FP32 heat tile (heat.rows 10 / 12): every size class --
+    grids of a few rows (several boundary rows in one wave), one and several
+    tiles along j, one and several along k, first and last planes -- with A
+    and B of different boundaries, bitwise the plain sweeps."""
+    A, B = grids("heat_3d", n, "f32", 7 * n + nsweeps)
+    pa, pb = run(st, "heat_3d", A, B, "f32", nsweeps, first, 1)
+    heat_rows(rows)
+    print(f"heat wave tile rows={rows} n={n} sweeps={nsweeps} first={first}", flush=True)
+    ta, tb = run(st, "heat_3d", A, B, "f32", nsweeps, first, 2)
+    assert same(ta, pa) and same(tb, pb)
+
+
+@pytest.mark.parametrize("rows", [10, 12])
+def test_plan_batch_wave_tiles(st, heat_rows, rows):
+    from npbench_amd import batch, device
+    heat_rows(rows)
+    cases = _mixed_batch("f32", 11)
+    for seg in (3, 32):
+        print(f"plan batch wave tiles rows={rows} seg={seg}", flush=True)
+        gs = []
+        for kind, A, B, ns, first in cases:
+            a, b = device.to_device(A, torch.float32), device.to_device(B, torch.float32)
+            gs.append(batch.Grid(kind, a, b, ns, first))
+        plan = batch.Plan(gs, tblock=2, seg=seg)
+        plan.launch()
+        torch.cuda.synchronize()
+        for g, (kind, A, B, ns, first) in zip(gs, cases):
+            pa, pb = run(st, kind, A, B, "f32", ns, first, 1)
+            assert same(g.A, pa) and same(g.B, pb), f"{kind} n={A.shape[0]}"
+        plan.close()
