@@ -326,7 +326,7 @@ __device__ __forceinline__ void heat_tile(const GridRound g, int tile,
 constexpr int kPrivateRows = 10;  // heat.rows >= this selects the wave-private tile
 constexpr bool wave_private(int rows) { return rows >= kPrivateRows; }
 
-// Raw buffer access to one plane (num_records = the plane's bytes; the
+// Raw buffer loads from one plane (num_records = the plane's bytes; the
 // descriptor's third word is the gfx9 raw-buffer format).  The 32-bit vector
 // offset is the lane's, the scalar offset the wave's row.
 struct Rsrc {
@@ -344,16 +344,6 @@ __device__ __forceinline__ void buf_load16(Rsrc rs, uint32_t voff, uint32_t soff
 }
 __device__ __forceinline__ float buf_load1(Rsrc rs, uint32_t voff, uint32_t soff) {
   return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs.r, voff, soff, 0));
-}
-__device__ __forceinline__ void buf_store16(Rsrc rs, uint32_t voff, uint32_t soff,
-                                            const float (&d)[4]) {
-  u32x4 v;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) v[i] = __float_as_uint(d[i]);
-  __builtin_amdgcn_raw_buffer_store_b128(v, rs.r, voff, soff, 0);
-}
-__device__ __forceinline__ void buf_store1(Rsrc rs, uint32_t voff, uint32_t soff, float x) {
-  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), rs.r, voff, soff, 0);
 }
 
 // Full-wave DPP shifts (wave_shr:1, wave_shl:1): lane l receives lane l-1's
@@ -375,14 +365,15 @@ __device__ __forceinline__ float lane_value(float x, int lane) {
 // only ever feed other such planes or boundary planes, and only level D's
 // planes inside the output segment are stored.
 //
-// The loop's memory accesses are unconditional, so the compiler waits for
-// exactly the loads a step consumes.  Stores that must not happen (rows past
-// the wave's output rows, planes outside the segment, lanes outside the
-// output columns) go through an empty descriptor or an out-of-range offset,
-// which the hardware drops; a stored vector that holds a global-boundary
-// column carries that column's current value of Y, rewritten unchanged.
+// The loop's loads are unconditional, so the compiler waits for exactly the
+// loads a step consumes.  Stores are whole 16-byte vectors from the lanes
+// whose vector holds output points (exec-masked global stores; buffer stores
+// of this loop lost data in a few lanes when two waves shared a SIMD); a
+// stored vector that holds a global-boundary column carries that column's
+// current value of Y, rewritten unchanged.
 // Level-0 planes and all patch values are loaded two steps ahead into two
-// register sets that the twice-unrolled loop uses in turn.
+// register sets that the twice-unrolled loop uses in turn.  The register
+// footprint (over 256 VGPRs) runs one wave per SIMD.
 template <int T, int JR, int D>
 __device__ __forceinline__ void heat_wave_tile(const GridRound g, int tile) {
   constexpr int V = 4;
@@ -390,7 +381,6 @@ __device__ __forceinline__ void heat_wave_tile(const GridRound g, int tile) {
   constexpr int WJ = JR - 2 * T;  // output rows of a wave
   static_assert(WJ >= 1 && JR <= 32, "a wave holds 2T halo rows plus its output rows, <= 32");
   static_assert(D >= 1 && D <= T, "depth");
-  constexpr uint32_t kDrop = 0x80000000u;  // beyond every plane descriptor's range
   const int n = g.n;
   // the wave index, known to the compiler as wave-uniform: row bases and row
   // conditions then live in scalar registers and branches
@@ -412,7 +402,7 @@ __device__ __forceinline__ void heat_wave_tile(const GridRound g, int tile) {
   // outside the grid read the row's first vector).  Planes and rows are
   // clamped into the grid.  Lanes whose vector holds output points store it.
   const uint32_t lob = vlive ? static_cast<uint32_t>(k0) * 4u : 0u;
-  const uint32_t sob = (vlive && k0 + V > klo && k0 < khi) ? lob : kDrop;
+  const bool sto = vlive && k0 + V > klo && k0 < khi;
   const int64_t plane = static_cast<int64_t>(n) * n;
   const float* X = static_cast<const float*>(g.X);
   float* Y = static_cast<float*>(g.Y);
@@ -492,24 +482,15 @@ __device__ __forceinline__ void heat_wave_tile(const GridRound g, int tile) {
         part[l][r][v] = 0.0f;
       }
 
-  // One step: consumes f (loaded two steps ago), then refills it for step P+2.
+  // One step: consumes f (loaded two steps ago) and refills it for step P+2
+  // as soon as level 0 is done with it.  Level 0 computes straight from f.
   auto step = [&](int P, Ahead& f) {
-    float in[JR][V];  // the arriving plane of the current level
-#pragma unroll
-    for (int r = 0; r < JR; ++r)
-#pragma unroll
-      for (int v = 0; v < V; ++v) in[r][v] = f.pre[r][v];
-    patch_k(in, f.kq[0], 0, JR);
-    float kq[T], jrow[T][V], ky = f.ky;
-#pragma unroll
-    for (int m = 0; m < T; ++m) {
-      kq[m] = f.kq[m];
-#pragma unroll
-      for (int v = 0; v < V; ++v) jrow[m][v] = f.jrow[m][v];
-    }
-    fetch(P + 2, f);
+    patch_k(f.pre, f.kq[0], 0, JR);
+    const float ky = f.ky;
+    float lv[JR][V];  // the arriving plane of level 1 and up
 #pragma unroll
     for (int l = 0; l < D; ++l) {
+      float (&in)[JR][V] = l == 0 ? f.pre : lv;  // the arriving plane of level l
       const int m = l + 1;  // the level produced, at plane q
       const int q = P - m;
       float o[JR][V];
@@ -520,11 +501,11 @@ __device__ __forceinline__ void heat_wave_tile(const GridRound g, int tile) {
       if (m == D) {
         patch_k(o, ky, T, JR - T);
         const bool inseg = q >= i0 && q < i1;
-        const Rsrc ry = plane_rsrc(Y + static_cast<int64_t>(inseg ? q : i0) * plane, plane);
-        const Rsrc none = plane_rsrc(Y, 0);
+        float* yq = Y + static_cast<int64_t>(inseg ? q : i0) * plane;
 #pragma unroll
         for (int r = T; r < JR - T; ++r)
-          buf_store16((inseg && jl + r < jhi) ? ry : none, sob, rowoff(r), o[r]);
+          if (inseg && jl + r < jhi && sto)
+            store16(reinterpret_cast<float*>(reinterpret_cast<char*>(yq) + rowoff(r) + lob), o[r]);
       } else if (q == 0 || q == n - 1) {
         // a boundary plane of an intermediate level (first and last tiles
         // along i).  The wait for every load on this rare path keeps the
@@ -534,7 +515,7 @@ __device__ __forceinline__ void heat_wave_tile(const GridRound g, int tile) {
         for (int r = m; r < JR - m; ++r) ld(rb, r, o[r]);
         __builtin_amdgcn_s_waitcnt(0);
       } else {
-        patch_k(o, kq[m < T ? m : T - 1], m, JR - m);
+        patch_k(o, f.kq[m < T ? m : T - 1], m, JR - m);
         if (slow) {
           const Rsrc rb = plane_of(bsrc(m), q);
 #pragma unroll
@@ -546,7 +527,7 @@ __device__ __forceinline__ void heat_wave_tile(const GridRound g, int tile) {
           for (int r = m; r < JR - m; ++r)
             if (r == jb)
 #pragma unroll
-              for (int v = 0; v < V; ++v) o[r][v] = jrow[m < T ? m : T - 1][v];
+              for (int v = 0; v < V; ++v) o[r][v] = f.jrow[m < T ? m : T - 1][v];
         }
       }
       // level l's arrival (plane P - l) becomes its newest plane
@@ -562,11 +543,12 @@ __device__ __forceinline__ void heat_wave_tile(const GridRound g, int tile) {
           a[l][r][v] = in[r][v];
         }
       }
+      if (l == 0) fetch(P + 2, f);
       if (m < D) {
 #pragma unroll
         for (int r = m; r < JR - m; ++r)
 #pragma unroll
-          for (int v = 0; v < V; ++v) in[r][v] = o[r][v];
+          for (int v = 0; v < V; ++v) lv[r][v] = o[r][v];
       }
     }
   };
