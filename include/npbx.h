@@ -165,6 +165,10 @@ typedef struct npbx_round_spec {
   int64_t level0;
   int32_t depth;
   int32_t reserved;
+  /* FP32 heat only, optional: the grid's boundary columns packed by
+   * npbx_pack_cols (rounds then read them from here instead of from A and
+   * B); NULL otherwise.  Results do not depend on it. */
+  const void *cols;
 } npbx_round_spec;
 
 int npbx_round_depths(int64_t nsweeps, int32_t tblock, int32_t *depths,
@@ -174,12 +178,23 @@ int npbx_queue_destroy(void *queue);
 int npbx_rounds_launch(const npbx_round_spec *rounds, int32_t count,
                        int32_t tblock, int32_t seg, void *queue, void *stream);
 
+/* Boundary columns of an FP32 heat grid (columns 0 and n-1 of every row of A
+ * and of B), packed into `cols` of npbx_cols_elems(n) = 4*n*n floats, for
+ * npbx_round_spec.cols.  Pack again after A's or B's boundary changes.  No
+ * counterpart in the reference (a layout for the batched/serving forms). */
+int npbx_cols_elems(int64_t n, int64_t *elems);
+int npbx_pack_cols(int dtype, const void *A, const void *B, int64_t n,
+                   void *cols, void *stream);
+
 /* ---- tuning ------------------------------------------------------------------
  * Process-wide knobs read when tile launches are planned (the autotuner in
  * npbench_amd/tune.py sets them): "heat.rows" = rows per wave of a heat tile
  * (2, 4, 6, 8; built for T=2: 2..8 (FP32), 2..6 (FP64), 2 and 4 (BF16); T=3:
- * 4 and 6; T=4: 4), "heat.seg" =
- * default planes per heat tile.  Results do not depend on them. */
+ * 4 and 6; T=4: 4; and for FP32 with T=2, 10 and 12, which select the
+ * wave-private heat tile: rows per wave including its own 2T halo rows),
+ * "heat.seg" = default planes per heat tile, "sweep.shape" = the plain
+ * sweep's planes per workgroup / planes loaded ahead (0: 16/4, 1: 16/8,
+ * 2: 32/8, 3: 8/4).  Results do not depend on them. */
 int npbx_set_tuning(const char *key, int64_t value);
 int npbx_get_tuning(const char *key, int64_t *value);
 

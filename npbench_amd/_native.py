@@ -40,7 +40,7 @@ class RoundSpec(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("dtype", ctypes.c_int32), ("n", ctypes.c_int64),
                 ("A", ctypes.c_void_p), ("B", ctypes.c_void_p), ("X", ctypes.c_void_p),
                 ("Y", ctypes.c_void_p), ("level0", ctypes.c_int64), ("depth", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("reserved", ctypes.c_int32), ("cols", ctypes.c_void_p)]
 
 
 _vp = ctypes.c_void_p
@@ -83,6 +83,8 @@ SIGNATURES = {
     "npbx_queue_destroy": [_vp],
     "npbx_rounds_launch": [ctypes.POINTER(RoundSpec), ctypes.c_int32, ctypes.c_int32,
                            ctypes.c_int32, _vp, _vp],
+    "npbx_cols_elems": [_i64, ctypes.POINTER(_i64)],
+    "npbx_pack_cols": [_int, _vp, _vp, _i64, _vp, _vp],
     "npbx_set_tuning": [ctypes.c_char_p, _i64],
     "npbx_get_tuning": [ctypes.c_char_p, ctypes.POINTER(_i64)],
     "npbx_hdiff": [_int, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _vp],

@@ -26,6 +26,9 @@ struct GridRound {
   int32_t tiles_j;  // tiles along the row axis
   int32_t seg;      // heat: output planes per tile along i
   int32_t rows;     // heat: rows per wave (a tile is 4*rows rows)
+  // FP32 heat: packed boundary columns of A and B (npbx_pack_cols layout
+  // [array][plane][row][2]), or null to read them from A and B directly
+  const void* cols;
 };
 
 // Up to kMaxBatch grid rounds passed by value as one launch's kernel

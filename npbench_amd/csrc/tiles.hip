@@ -430,12 +430,20 @@ __device__ __forceinline__ void heat_wave_tile(const GridRound g, int tile) {
 
   // Boundary columns: lane-spread values (lane r < JR: row r's column 0;
   // lane 32 + r: its column n-1), patched into the lanes that hold them.
-  const int kr = lane & 31;
-  const int kcol = lane >= 32 ? n - 1 : 0;
+  // They come from the packed copy when there is one (the wave's values of a
+  // plane are 2 x JR consecutive floats), else from the arrays themselves
+  // (one cache line per value).
+  const int kr = lane & 31, side = lane >= 32 ? 1 : 0;
+  const int krow = min(max(jl + kr, 0), n - 1);
   const bool is_k0 = vlive && k0 == 0, is_kn = vlive && k0 + V == n;
-  const uint32_t kboff =
-      (static_cast<uint32_t>(min(max(jl + kr, 0), n - 1) * n) + static_cast<uint32_t>(kcol)) * 4u;
-  auto load_cols = [&](const float* base, int q) { return buf_load1(plane_of(base, q), kboff, 0u); };
+  const float* cols = static_cast<const float*>(g.cols);
+  auto load_cols = [&](const float* base, int q) {
+    const int64_t qq = min(max(q, 0), n - 1);
+    const int64_t arr = base == gB ? 1 : 0;
+    const float* p = cols ? cols + ((arr * n + qq) * n + krow) * 2 + side
+                          : base + qq * plane + static_cast<int64_t>(krow) * n + (side ? n - 1 : 0);
+    return *p;
+  };
   auto patch_k = [&](float (&x)[JR][V], float kb, int rlo, int rhi) {
 #pragma unroll
     for (int r = 0; r < JR; ++r) {
@@ -465,7 +473,8 @@ __device__ __forceinline__ void heat_wave_tile(const GridRound g, int tile) {
 #pragma unroll
     for (int m = 1; m < D; ++m) {
       f.kq[m] = load_cols(bsrc(m), S - m);
-      ld(plane_of(bsrc(m), S - m), jbr, f.jrow[m]);
+      // without a boundary row this reads a line the level-0 loads just read
+      ld(jb >= 0 ? plane_of(bsrc(m), S - m) : rx, jbr, f.jrow[m]);
     }
     f.ky = load_cols(Y, S - D);
   };
@@ -793,6 +802,41 @@ std::atomic<int> g_heat_rows{4};
 std::atomic<int> g_heat_seg{32};
 std::atomic<int> g_sweep_shape{0};
 
+// Packed boundary columns of a heat grid: cols[((a*n + q)*n + j)*2 + s] =
+// (a ? B : A)[q][j][s ? n-1 : 0].  The FP32 wave-private tile reads its
+// boundary patches from here: 2 x rows consecutive floats per wave and plane
+// instead of one cache line per value.
+__global__ __launch_bounds__(256) void pack_cols_kernel(const float* __restrict__ A,
+                                                        const float* __restrict__ B,
+                                                        float* __restrict__ cols, int n) {
+  const int64_t nn = static_cast<int64_t>(n) * n;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; t < nn;
+       t += static_cast<int64_t>(gridDim.x) * 256) {
+    const int64_t row = t * n;  // start of row j of plane q, t = q*n + j
+    cols[t * 2] = A[row];
+    cols[t * 2 + 1] = A[row + n - 1];
+    cols[(nn + t) * 2] = B[row];
+    cols[(nn + t) * 2 + 1] = B[row + n - 1];
+  }
+}
+
+int64_t cols_elems(int64_t n) { return 4 * n * n; }
+
+int pack_cols(const void* A, const void* B, int n, void* cols, hipStream_t stream) {
+  const int64_t nn = static_cast<int64_t>(n) * n;
+  const unsigned blocks = static_cast<unsigned>(std::min<int64_t>(cdiv(nn, 256), 4096));
+  hipLaunchKernelGGL(pack_cols_kernel, dim3(blocks), dim3(256), 0, stream,
+                     static_cast<const float*>(A), static_cast<const float*>(B),
+                     static_cast<float*>(cols), n);
+  NPBX_LAUNCHED("pack_cols");
+  return NPBX_OK;
+}
+
+// Whether a grid's rounds run the wave-private tile (and so use packed columns).
+bool uses_cols(int kind, int dtype, int rows) {
+  return kind == kHeat3D && dtype == NPBX_F32 && wave_private(rows);
+}
+
 int max_depth(int dtype) {
   switch (dtype) {
     case NPBX_F64: return MaxDepth<double>::value;
@@ -929,6 +973,12 @@ int tiled_sweeps(int kind, int dtype, void* A, void* B, int n, int64_t first, in
   g.n = n;
   g.kind = kind;
   tile_grid(g, kind, n, dtype, T, seg > 0 ? seg : g_heat_seg.load());
+  if (uses_cols(kind, dtype, g.rows)) {
+    void* ws = nullptr;
+    NPBX_TRY(workspace(static_cast<size_t>(cols_elems(n)) * sizeof(float), &ws));
+    NPBX_TRY(pack_cols(A, B, n, ws, stream));
+    g.cols = ws;
+  }
   const int64_t ntiles = tiles_per_round(g);
   NPBX_REQUIRE(ntiles < (int64_t(1) << 31), "too many tiles");
   int64_t level = first;
@@ -1004,12 +1054,21 @@ int64_t grid_lups(const GridRound& g, int64_t sweeps) {
   return sweeps * (g.kind == kHeat3D ? inner * inner * inner : inner * inner);
 }
 
+struct ColsPack {
+  const void* A;
+  const void* B;
+  int n;
+  int64_t off;  // elements into Plan::d_cols
+};
+
 struct Plan {
   int dtype = 0;
   int T = 0;
   int rows = 4;
   int nblocks = 0;
   std::vector<RoundBatch> launches;
+  std::vector<ColsPack> packs;  // grids whose boundary columns each launch packs
+  float* d_cols = nullptr;
   QueueState* d_queue = nullptr;
   int64_t ntiles = 0, lups = 0;
   hipGraph_t graph = nullptr;
@@ -1022,11 +1081,15 @@ int plan_free(Plan* p) {
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   if (p->d_queue) (void)hipFree(p->d_queue);
+  if (p->d_cols) (void)hipFree(p->d_cols);
   delete p;
   return NPBX_OK;
 }
 
 int plan_enqueue(Plan* p, hipStream_t s) {
+  // A and B may have been rewritten since the last launch: pack their
+  // boundary columns afresh (one small kernel per grid, before the rounds).
+  for (const ColsPack& c : p->packs) NPBX_TRY(pack_cols(c.A, c.B, c.n, p->d_cols + c.off, s));
   for (const RoundBatch& b : p->launches) {
     const int nt = b.cum[b.count];
     if (nt == 0) continue;
@@ -1072,6 +1135,23 @@ extern "C" int npbx_plan_create(const npbx_grid_spec* specs, int32_t count, int3
   p->dtype = dtype;
   p->T = tblock;
   p->lups = lups;
+  int64_t cols_total = 0;
+  for (int i = 0; i < count; ++i)
+    if (uses_cols(base[i].kind, dtype, base[i].rows)) {
+      p->packs.push_back({specs[i].A, specs[i].B, base[i].n, cols_total});
+      cols_total += cols_elems(base[i].n);
+    }
+  if (cols_total > 0) {
+    const hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->d_cols),
+                                   static_cast<size_t>(cols_total) * sizeof(float));
+    if (e != hipSuccess) {
+      plan_free(p);
+      return fail(NPBX_ENOMEM, "plan_create: %s", hipGetErrorString(e));
+    }
+    size_t k = 0;
+    for (int i = 0; i < count; ++i)
+      if (uses_cols(base[i].kind, dtype, base[i].rows)) base[i].cols = p->d_cols + p->packs[k++].off;
+  }
   std::vector<int64_t> level(count);
   std::vector<void*> X(count);
   for (int i = 0; i < count; ++i) {
@@ -1160,6 +1240,21 @@ extern "C" int npbx_round_depths(int64_t nsweeps, int32_t tblock, int32_t* depth
   return NPBX_OK;
 }
 
+extern "C" int npbx_cols_elems(int64_t n, int64_t* elems) {
+  NPBX_REQUIRE(elems && n >= 0, "cols_elems: bad arguments");
+  *elems = cols_elems(n);
+  return NPBX_OK;
+}
+
+extern "C" int npbx_pack_cols(int dtype, const void* A, const void* B, int64_t n, void* cols,
+                              void* stream) {
+  NPBX_REQUIRE(dtype == NPBX_F32, "pack_cols: packed columns are FP32 only (got %s)",
+               dtype_name(dtype));
+  NPBX_REQUIRE(n >= 3 && n <= 2048, "pack_cols: n=%lld out of range", static_cast<long long>(n));
+  NPBX_REQUIRE(A && B && cols, "pack_cols: null array");
+  return pack_cols(A, B, static_cast<int>(n), cols, as_stream(stream));
+}
+
 extern "C" int npbx_queue_create(void** queue) {
   NPBX_REQUIRE(queue, "queue_create: null output pointer");
   QueueState* q = nullptr;
@@ -1192,6 +1287,7 @@ extern "C" int npbx_rounds_launch(const npbx_round_spec* specs, int32_t count, i
     NPBX_REQUIRE(s.depth >= 1 && s.depth <= tblock && s.level0 >= 0,
                  "rounds_launch: grid %d depth %d outside [1, %d]", i, s.depth, tblock);
     tile_grid(g, g.kind, g.n, dtype, tblock, seg);
+    if (uses_cols(g.kind, dtype, g.rows)) g.cols = s.cols;
     g.X = s.X;
     g.Y = s.Y;
     g.parity0 = static_cast<int32_t>(s.level0 & 1);

@@ -60,7 +60,9 @@ class Request:
         return self.sweeps * (self.n - 2) ** self.ndim
 
     def footprint(self, dtype):
-        return 2 * self.n ** self.ndim * ELEM[dtype]
+        """Grid bytes plus, for FP32 heat, the packed boundary columns."""
+        cols = 16 * self.n ** 2 if (self.kind == "heat_3d" and dtype == torch.float32) else 0
+        return 2 * self.n ** self.ndim * ELEM[dtype] + cols
 
 
 def make_inputs(req, dtype, device="cuda"):
@@ -107,11 +109,12 @@ def round_schedule(nsweeps, tblock):
 
 
 class _Active:
-    __slots__ = ("req", "A", "B", "X", "level", "rounds", "next_round", "admitted")
+    __slots__ = ("req", "A", "B", "cols", "X", "level", "rounds", "next_round", "admitted")
 
-    def __init__(self, req, A, B, tblock, admitted):
+    def __init__(self, req, A, B, tblock, admitted, cols=None):
         self.req = req
         self.A, self.B = A, B
+        self.cols = cols  # packed boundary columns (FP32 heat), or None
         self.level = 0
         self.X = A  # level 0 lives in A
         self.rounds = round_schedule(req.sweeps, tblock)
@@ -190,6 +193,7 @@ class Server:
             s.X, s.Y = dev.ptr(a.X), dev.ptr(Y)
             s.level0 = a.level
             s.depth = depth
+            s.cols = dev.ptr(a.cols) if a.cols is not None else None
             a.X = Y
             a.level += depth
             a.next_round += 1
@@ -243,9 +247,18 @@ class Server:
                         break
                     waiting.popleft()
                     A, B = make_inputs(req, self.dtype)
+                    cols = None
+                    if req.kind == "heat_3d" and self.dtype == torch.float32:
+                        # boundary columns packed once per request (A and B's
+                        # boundaries never change while it is served)
+                        n_el = ctypes.c_int64()
+                        nat.call("npbx_cols_elems", req.n, ctypes.byref(n_el))
+                        cols = torch.empty(n_el.value, dtype=torch.float32, device="cuda")
+                        nat.call("npbx_pack_cols", self.code, dev.ptr(A), dev.ptr(B), req.n,
+                                 dev.ptr(cols), dev.stream_handle(self.stream))
                     used += fp
                     self.peak_used = max(self.peak_used, used)
-                    active.append(_Active(req, A, B, self.tblock, now()))
+                    active.append(_Active(req, A, B, self.tblock, now(), cols))
                 if active:
                     if len(inflight) >= self.inflight:
                         retire(True)

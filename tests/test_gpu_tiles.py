@@ -219,3 +219,24 @@ def test_plan_batch_wave_tiles(st, heat_rows, rows):
             pa, pb = run(st, kind, A, B, "f32", ns, first, 1)
             assert same(g.A, pa) and same(g.B, pb), f"{kind} n={A.shape[0]}"
         plan.close()
+
+
+def test_pack_cols_layout(st):
+    """npbx_pack_cols: cols[a][q][j][s] = (A, B)[a][q][j][(0, n-1)[s]]."""
+    import ctypes
+    from npbench_amd import _native, device
+    n = 20
+    rng = np.random.default_rng(3)
+    a = device.to_device(rng.random((n, n, n)), torch.float32)
+    b = device.to_device(rng.random((n, n, n)), torch.float32)
+    elems = ctypes.c_int64()
+    _native.call("npbx_cols_elems", n, ctypes.byref(elems))
+    assert elems.value == 4 * n * n
+    cols = torch.empty(elems.value, dtype=torch.float32, device="cuda")
+    _native.call("npbx_pack_cols", device.dtype_code(a), a.data_ptr(), b.data_ptr(), n,
+                 cols.data_ptr(), None)
+    torch.cuda.synchronize()
+    want = torch.stack([torch.stack([x[:, :, 0], x[:, :, n - 1]], dim=-1) for x in (a, b)])
+    assert torch.equal(cols.view(2, n, n, 2), want)
+    with pytest.raises(_native.NpbxError):
+        _native.call("npbx_pack_cols", 0, a.data_ptr(), b.data_ptr(), n, cols.data_ptr(), None)
