@@ -41,6 +41,7 @@ TSTEPS = 101
 SWEEPS = 2 * (TSTEPS - 1)
 BATCH = 8
 SIZE = {"f64": 8, "f32": 4, "bf16": 2}
+SERVE_HEAT_ROWS = 12  # serving's heat tiles: the wave-private FP32 tile
 
 
 def cpu_model():
@@ -73,11 +74,26 @@ class Clock:
         return time.perf_counter() - t0, self.timer.elapsed_ms() / 1e3
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of ``kernel`` on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None when there is none)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    return None if rec is None else int(rec["read_bytes"] + rec["write_bytes"])
+
+
 def roofline(bytes_per_launch, launch_s, kernel):
     achieved = bytes_per_launch / launch_s / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": kernel, "avg_launch_us": round(launch_s * 1e6, 3)}
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(kernel), "kernel": kernel,
+            "avg_launch_us": round(launch_s * 1e6, 3)}
 
 
 def seeded_grid(n):
@@ -111,7 +127,8 @@ def batched_fp32(st, dev, batch, tune, torch, clock, args, A64):
             plan.capture(side)
             clock = type(clock)(dev, side)
         launches = plan.info()["launches"]
-        kernel = f"stencil_tiles<float,T={cfg['T']},R={cfg['rows']}>"
+        kind = "wave-private" if cfg["rows"] >= 10 else "shared"
+        kernel = f"stencil_tiles<float,T={cfg['T']},R={cfg['rows']}> ({kind} heat tile)"
 
         def step():
             plan.launch(clock.stream)
@@ -120,10 +137,12 @@ def batched_fp32(st, dev, batch, tune, torch, clock, args, A64):
     wall, ev = clock.run(step, args.steps)
     # algorithmic bytes: one read and one write of every grid per round trip
     bytes_per_lup = 2 * SIZE["f32"] / cfg["T"]
-    out = {"cfg": cfg, "wall": wall, "event": ev,
-           "glups": lup_step * args.steps / wall / 1e9,
-           "roofline": roofline(bytes_per_lup * lup_step / launches,
-                                ev / (args.steps * launches), kernel),
+    glups = lup_step * args.steps / wall / 1e9
+    rl = roofline(bytes_per_lup * lup_step / launches, ev / (args.steps * launches), kernel)
+    # T-independent view: the bandwidth a plain sweep would need for this rate
+    rl["effective_GBps"] = round(glups * 2 * SIZE["f32"], 1)
+    rl["bytes_per_lup"] = bytes_per_lup
+    out = {"cfg": cfg, "wall": wall, "event": ev, "glups": glups, "roofline": rl,
            "launches_per_step": launches}
     if plan is not None:
         plan.close()
@@ -159,8 +178,9 @@ def fp32_error(st, dev, torch, A64, B64):
 
 
 # ----------------------------------------------------------------- serving
-def serving(torch, args):
-    from npbench_amd import serve
+def serving(torch, args, heat_rows=SERVE_HEAT_ROWS):
+    from npbench_amd import _native as nat, serve
+    nat.set_tuning("heat.rows", heat_rows)
     reqs = serve.poisson_mix(args.serve_requests, args.serve_qps, seed=42)
     srv = serve.Server(dtype=torch.float32, tblock=2, hbm_budget=32 << 30, max_active=48)
     srv.run(serve.poisson_mix(8, 1e6, seed=7), realtime=False)  # warm-up
@@ -170,7 +190,7 @@ def serving(torch, args):
     out = serve.summarize(done, wall)
     out.update({"qps": args.serve_qps, "mix": "50% jacobi_2d side log-U[128,4096], 50% heat_3d "
                 "side U[32,256], steps U[10,200], FP32, seed 42", "tblock": 2,
-                "peak_active": srv.peak_active, "ticks": srv.ticks})
+                "heat_rows": heat_rows, "peak_active": srv.peak_active, "ticks": srv.ticks})
     srv.close()
     return out
 
