@@ -371,16 +371,19 @@ __device__ __forceinline__ float lane_value(float x, int lane) {
 // of this loop lost data in a few lanes when two waves shared a SIMD); a
 // stored vector that holds a global-boundary column carries that column's
 // current value of Y, rewritten unchanged.
-// Level-0 planes and all patch values are loaded two steps ahead into two
-// register sets that the twice-unrolled loop uses in turn.  The register
-// footprint (over 256 VGPRs) runs one wave per SIMD.
-template <int T, int JR, int D>
+// Level-0 planes and all patch values are loaded AH steps ahead: with AH = 2
+// into two register sets that the twice-unrolled loop uses in turn, with
+// AH = 1 into one set refilled as soon as level 0 is done with it (deeper
+// tiles, whose longer steps cover the latency).  The register footprint
+// (over 256 VGPRs) runs one wave per SIMD.
+template <int T, int JR, int D, int AH>
 __device__ __forceinline__ void heat_wave_tile(const GridRound g, int tile) {
   constexpr int V = 4;
   constexpr int KT = 64 * V;
   constexpr int WJ = JR - 2 * T;  // output rows of a wave
   static_assert(WJ >= 1 && JR <= 32, "a wave holds 2T halo rows plus its output rows, <= 32");
   static_assert(D >= 1 && D <= T, "depth");
+  static_assert(AH == 1 || AH == 2, "look-ahead");
   const int n = g.n;
   // the wave index, known to the compiler as wave-uniform: row bases and row
   // conditions then live in scalar registers and branches
@@ -491,11 +494,19 @@ __device__ __forceinline__ void heat_wave_tile(const GridRound g, int tile) {
         part[l][r][v] = 0.0f;
       }
 
-  // One step: consumes f (loaded two steps ago) and refills it for step P+2
+  // One step: consumes f (loaded AH steps ago) and refills it for step P+AH
   // as soon as level 0 is done with it.  Level 0 computes straight from f.
   auto step = [&](int P, Ahead& f) {
     patch_k(f.pre, f.kq[0], 0, JR);
+    // patch values used after the refill below (levels 2 and up, and Y's)
     const float ky = f.ky;
+    float kq[T], jrow[T][V];
+#pragma unroll
+    for (int m = 1; m < T; ++m) {
+      kq[m] = f.kq[m];
+#pragma unroll
+      for (int v = 0; v < V; ++v) jrow[m][v] = f.jrow[m][v];
+    }
     float lv[JR][V];  // the arriving plane of level 1 and up
 #pragma unroll
     for (int l = 0; l < D; ++l) {
@@ -524,7 +535,7 @@ __device__ __forceinline__ void heat_wave_tile(const GridRound g, int tile) {
         for (int r = m; r < JR - m; ++r) ld(rb, r, o[r]);
         __builtin_amdgcn_s_waitcnt(0);
       } else {
-        patch_k(o, f.kq[m < T ? m : T - 1], m, JR - m);
+        patch_k(o, kq[m < T ? m : T - 1], m, JR - m);
         if (slow) {
           const Rsrc rb = plane_of(bsrc(m), q);
 #pragma unroll
@@ -536,7 +547,7 @@ __device__ __forceinline__ void heat_wave_tile(const GridRound g, int tile) {
           for (int r = m; r < JR - m; ++r)
             if (r == jb)
 #pragma unroll
-              for (int v = 0; v < V; ++v) o[r][v] = f.jrow[m < T ? m : T - 1][v];
+              for (int v = 0; v < V; ++v) o[r][v] = jrow[m < T ? m : T - 1][v];
         }
       }
       // level l's arrival (plane P - l) becomes its newest plane
@@ -552,7 +563,7 @@ __device__ __forceinline__ void heat_wave_tile(const GridRound g, int tile) {
           a[l][r][v] = in[r][v];
         }
       }
-      if (l == 0) fetch(P + 2, f);
+      if (l == 0) fetch(P + AH, f);
       if (m < D) {
 #pragma unroll
         for (int r = m; r < JR - m; ++r)
@@ -563,26 +574,33 @@ __device__ __forceinline__ void heat_wave_tile(const GridRound g, int tile) {
   };
 
   const int pbeg = max(i0 - D, 0), pfin = i1 - 1 + D;
-  Ahead fa, fb;
-  fetch(pbeg, fa);
-  fetch(pbeg + 1, fb);
-  for (int P = pbeg; P <= pfin; P += 2) {
-    step(P, fa);
-    if (P + 1 > pfin) break;
-    step(P + 1, fb);
+  if constexpr (AH == 2) {
+    Ahead fa, fb;
+    fetch(pbeg, fa);
+    fetch(pbeg + 1, fb);
+    for (int P = pbeg; P <= pfin; P += 2) {
+      step(P, fa);
+      if (P + 1 > pfin) break;
+      step(P + 1, fb);
+    }
+  } else {
+    Ahead fa;
+    fetch(pbeg, fa);
+    for (int P = pbeg; P <= pfin; ++P) step(P, fa);
   }
 }
 
 // Dispatches a round's runtime depth to the compile-time one.
 template <int T, int JR>
 __device__ __forceinline__ void heat_wave_round(const GridRound g, int tile) {
+  constexpr int AH = T >= 3 ? 1 : 2;
   if constexpr (T >= 2) {
-    if (g.depth == 2) return heat_wave_tile<T, JR, 2>(g, tile);
+    if (g.depth == 2) return heat_wave_tile<T, JR, 2, AH>(g, tile);
   }
   if constexpr (T >= 3) {
-    if (g.depth == 3) return heat_wave_tile<T, JR, 3>(g, tile);
+    if (g.depth == 3) return heat_wave_tile<T, JR, 3, AH>(g, tile);
   }
-  heat_wave_tile<T, JR, 1>(g, tile);
+  heat_wave_tile<T, JR, 1, AH>(g, tile);
 }
 
 // ---------------------------------------------------------------- jacobi 2-D
@@ -919,6 +937,8 @@ int with_shape(int T, int RH, F&& f) {
   if constexpr (std::is_same_v<S, float>) {  // wave-private heat tiles
     if (T == 2 && RH == 10) return f.template operator()<2, 10>();
     if (T == 2 && RH == 12) return f.template operator()<2, 12>();
+    if (T == 3 && RH == 12) return f.template operator()<3, 12>();
+    if (T == 3 && RH == 14) return f.template operator()<3, 14>();
   }
   return fail(NPBX_EUNSUPPORTED, "tile depth T=%d with %d heat rows per wave is not built for "
               "this dtype", T, RH);
@@ -1309,8 +1329,8 @@ extern "C" int npbx_set_tuning(const char* key, int64_t value) {
   NPBX_REQUIRE(key, "set_tuning: null key");
   if (std::strcmp(key, "heat.rows") == 0) {
     NPBX_REQUIRE(value == 2 || value == 4 || value == 6 || value == 8 || value == 10 ||
-                     value == 12,
-                 "set_tuning: heat.rows must be 2, 4, 6, 8 (shared tile) or 10, 12 "
+                     value == 12 || value == 14,
+                 "set_tuning: heat.rows must be 2, 4, 6, 8 (shared tile) or 10, 12, 14 "
                  "(wave-private FP32 tile)");
     g_heat_rows.store(static_cast<int>(value));
     return NPBX_OK;

@@ -30,9 +30,10 @@ DEFAULT_CACHE = pathlib.Path(os.environ.get(
     "NPBX_TUNE_CACHE", pathlib.Path.home() / ".cache" / "npbench_amd" / "tune.json"))
 
 # (T, rows) pairs instantiated in libnpbx (rows only matters for heat; FP32
-# rows 10 and 12 select the wave-private heat tile)
+# rows 10 .. 14 select the wave-private heat tile)
 SHAPES = {
-    torch.float32: [(2, 2), (2, 4), (2, 6), (2, 8), (3, 4), (3, 6), (4, 4), (2, 10), (2, 12)],
+    torch.float32: [(2, 2), (2, 4), (2, 6), (2, 8), (3, 4), (3, 6), (4, 4), (2, 10), (2, 12),
+                    (3, 12), (3, 14)],
     torch.float64: [(2, 2), (2, 4), (2, 6)],
     torch.bfloat16: [(2, 2), (2, 4)],
 }

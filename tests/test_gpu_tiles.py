@@ -185,34 +185,38 @@ def heat_rows():
 WAVE_HEAT_CASES = [4, 8, 12, 20, 68, 264, 520]
 
 
-@pytest.mark.parametrize("rows", [10, 12])
+WAVE_SHAPES = [(2, 10), (2, 12), (3, 12), (3, 14)]  # (T, rows)
+
+
+@pytest.mark.parametrize("T,rows", WAVE_SHAPES)
 @pytest.mark.parametrize("n", WAVE_HEAT_CASES)
 @pytest.mark.parametrize("nsweeps,first", [(1, 1), (2, 0), (3, 0), (6, 1), (11, 0)])
-def test_heat_wave_tiles_match_plain(st, heat_rows, rows, n, nsweeps, first):
-    """The wave-private FP32 heat tile (heat.rows 10 / 12): every size class --
-    grids of a few rows (several boundary rows in one wave), one and several
-    tiles along j, one and several along k, first and last planes -- with A
-    and B of different boundaries, bitwise the plain sweeps."""
+def test_heat_wave_tiles_match_plain(st, heat_rows, T, rows, n, nsweeps, first):
+    """The wave-private FP32 heat tile (heat.rows 10 .. 14, depth 2 and 3):
+    every size class -- grids of a few rows (several boundary rows in one
+    wave), one and several tiles along j, one and several along k, first and
+    last planes, rounds of every depth -- with A and B of different
+    boundaries, bitwise the plain sweeps."""
     A, B = grids("heat_3d", n, "f32", 7 * n + nsweeps)
     pa, pb = run(st, "heat_3d", A, B, "f32", nsweeps, first, 1)
     heat_rows(rows)
-    print(f"heat wave tile rows={rows} n={n} sweeps={nsweeps} first={first}", flush=True)
-    ta, tb = run(st, "heat_3d", A, B, "f32", nsweeps, first, 2)
+    print(f"heat wave tile T={T} rows={rows} n={n} sweeps={nsweeps} first={first}", flush=True)
+    ta, tb = run(st, "heat_3d", A, B, "f32", nsweeps, first, T)
     assert same(ta, pa) and same(tb, pb)
 
 
-@pytest.mark.parametrize("rows", [10, 12])
-def test_plan_batch_wave_tiles(st, heat_rows, rows):
+@pytest.mark.parametrize("T,rows", WAVE_SHAPES)
+def test_plan_batch_wave_tiles(st, heat_rows, T, rows):
     from npbench_amd import batch, device
     heat_rows(rows)
     cases = _mixed_batch("f32", 11)
     for seg in (3, 32):
-        print(f"plan batch wave tiles rows={rows} seg={seg}", flush=True)
+        print(f"plan batch wave tiles T={T} rows={rows} seg={seg}", flush=True)
         gs = []
         for kind, A, B, ns, first in cases:
             a, b = device.to_device(A, torch.float32), device.to_device(B, torch.float32)
             gs.append(batch.Grid(kind, a, b, ns, first))
-        plan = batch.Plan(gs, tblock=2, seg=seg)
+        plan = batch.Plan(gs, tblock=T, seg=seg)
         plan.launch()
         torch.cuda.synchronize()
         for g, (kind, A, B, ns, first) in zip(gs, cases):
