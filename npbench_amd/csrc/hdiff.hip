@@ -7,9 +7,10 @@
 // (x, y, k) sits over in-field point (x+2, y+2, k).
 //
 // Marching kernel (hdiff_march): a lane owns one output column y and one
-// 16-byte vector of V consecutive k (lanes enumerate (y, k-vector) pairs,
+// vector of V consecutive k (16 bytes; 8 for BF16, see march_vec) (lanes
+// enumerate (y, k-vector) pairs,
 // so any K maps densely onto waves and loads stay contiguous), and walks
-// down a strip of kStrip output rows x.  It keeps in registers the in-field
+// down a strip of march_strip<S>() output rows x.  It keeps in registers the in-field
 // rows p-2..p+2 of its column q, rows p-1..p+1 of columns q+-1 and row p of
 // q+-2 (p = x+2, q = y+2), and carries lap(p-1), lap(p) and the i-flux of
 // the face below from one row to the next, so each output point loads five
@@ -113,7 +114,17 @@ __global__ __launch_bounds__(256) void hdiff_copy_halo(const S* __restrict__ src
   }
 }
 
-constexpr int kStrip = 16;  // output rows per thread
+// Output rows per thread: 16, and 8 for BF16, whose loads carry half the
+// bytes and which needs the doubled number of waves to hide their latency
+// (FP32 measured slower with 8).
+template <class S>
+constexpr int march_strip() { return sizeof(S) == 2 ? 8 : 16; }
+
+// k-elements per lane of the marching kernel: one 16-byte vector, except
+// BF16, which takes 4 (8 bytes) so that it runs as many waves with the same
+// register footprint as FP32 (8 would halve the waves and double the window).
+template <class S>
+constexpr int march_vec() { return sizeof(S) == 2 ? 4 : Vec16<S>::N; }
 
 template <class S>
 __global__ __launch_bounds__(256) void hdiff_march(const S* __restrict__ in, S* __restrict__ out,
@@ -121,18 +132,19 @@ __global__ __launch_bounds__(256) void hdiff_march(const S* __restrict__ in, S* 
                                                    int K, int xblocks, S* __restrict__ halo_dst) {
   using T = Storage<S>;
   using C = typename T::C;
-  constexpr int V = Vec16<S>::N;
+  constexpr int V = march_vec<S>();
   const int b = xcd_tile(blockIdx.x, gridDim.x);
   const int xb = b % xblocks, strip = b / xblocks;
   const int kvs = K / V;
   const int t = xb * 256 + threadIdx.x;
   if (t >= J * kvs) return;
   const int y = t / kvs, k0 = (t % kvs) * V;
+  constexpr int kStrip = march_strip<S>();
   const int x0 = strip * kStrip, x1 = min(x0 + kStrip, I);
   const int64_t si = static_cast<int64_t>(J + 4) * K;  // in-field row stride
   const int64_t sj = K;
   const S* f = in + (y + 2) * sj + k0;                // column q = y+2, row 0
-  auto ld = [&](int p, int dq, C (&v)[V]) { load16(f + p * si + dq * sj, v); };
+  auto ld = [&](int p, int dq, C (&v)[V]) { loadv<S, V>(f + p * si + dq * sj, v); };
 
   // window at row p = x0+2: c0[0..4] = rows p-2..p+2 (col q), cm/cp[0..2] =
   // rows p-1..p+1 (cols q-1 / q+1), cmm / cpp = row p (cols q-2 / q+2)
@@ -153,7 +165,7 @@ __global__ __launch_bounds__(256) void hdiff_march(const S* __restrict__ in, S* 
   ld(p0, -2, nmm);
   ld(p0, 2, npp);
   const S* cfp = coeff + (static_cast<int64_t>(x0) * J + y) * K + k0;
-  load16(cfp, cf);
+  loadv<S, V>(cfp, cf);
 
   auto lap = [](C c, C ip, C im, C jp, C jm) {
     C nb = ip + im;
@@ -181,7 +193,7 @@ __global__ __launch_bounds__(256) void hdiff_march(const S* __restrict__ in, S* 
       ld(p + 2, 1, np);
       ld(p + 1, -2, nmm);
       ld(p + 1, 2, npp);
-      load16(cfp + static_cast<int64_t>(x + 1 - x0) * J * K, cf);
+      loadv<S, V>(cfp + static_cast<int64_t>(x + 1 - x0) * J * K, cf);
     }
     C res[V];
 #pragma unroll
@@ -210,9 +222,9 @@ __global__ __launch_bounds__(256) void hdiff_march(const S* __restrict__ in, S* 
 #pragma unroll
     for (int v = 0; v < V; ++v) o[v] = res[v];
     if (halo_dst)
-      store16(halo_dst + p * si + (y + 2) * sj + k0, o);
+      storev<S, V>(halo_dst + p * si + (y + 2) * sj + k0, o);
     else
-      store16(out + (static_cast<int64_t>(x) * J + y) * K + k0, o);
+      storev<S, V>(out + (static_cast<int64_t>(x) * J + y) * K + k0, o);
     // slide the window down one row
 #pragma unroll
     for (int v = 0; v < V; ++v) {
@@ -230,10 +242,10 @@ __global__ __launch_bounds__(256) void hdiff_march(const S* __restrict__ in, S* 
 template <class S>
 int run_hdiff(const S* in, S* out, const S* coeff, int I, int J, int K, int64_t nsteps,
               hipStream_t stream) {
-  constexpr int V = Vec16<S>::N;
+  constexpr int V = march_vec<S>();
   const bool vec = K % V == 0 && aligned16(in) && aligned16(out) && aligned16(coeff);
   const int xblocks = static_cast<int>(cdiv(static_cast<int64_t>(J) * (K / V), 256));
-  const int64_t mblocks = int64_t(xblocks) * cdiv(I, kStrip);
+  const int64_t mblocks = int64_t(xblocks) * cdiv(I, march_strip<S>());
   NPBX_REQUIRE(!vec || mblocks < (int64_t(1) << 31), "hdiff: grid too large");
   auto launch = [&](const S* src, S* next) -> int {
     if (vec) {

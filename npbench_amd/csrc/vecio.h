@@ -37,6 +37,37 @@ __device__ __forceinline__ void store16(S* __restrict__ p,
   *reinterpret_cast<uint4*>(p) = raw;
 }
 
+// V consecutive elements in one 8- or 16-byte access (V * sizeof(S)).
+template <class S, int V>
+__device__ __forceinline__ void loadv(const S* __restrict__ p, typename Storage<S>::C (&out)[V]) {
+  static_assert(V * sizeof(S) == 16 || V * sizeof(S) == 8, "8- or 16-byte vectors");
+  if constexpr (V * sizeof(S) == 16) {
+    load16(p, out);
+  } else {
+    const uint2 raw = *reinterpret_cast<const uint2*>(p);
+    S tmp[V];
+    __builtin_memcpy(tmp, &raw, 8);
+#pragma unroll
+    for (int v = 0; v < V; ++v) out[v] = Storage<S>::load(tmp[v]);
+  }
+}
+
+template <class S, int V>
+__device__ __forceinline__ void storev(S* __restrict__ p,
+                                       const typename Storage<S>::C (&in)[V]) {
+  static_assert(V * sizeof(S) == 16 || V * sizeof(S) == 8, "8- or 16-byte vectors");
+  if constexpr (V * sizeof(S) == 16) {
+    store16(p, in);
+  } else {
+    S tmp[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) tmp[v] = Storage<S>::store(in[v]);
+    uint2 raw;
+    __builtin_memcpy(&raw, tmp, 8);
+    *reinterpret_cast<uint2*>(p) = raw;
+  }
+}
+
 inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
 // Workgroups of a 1-D grid are handed to the 8 XCDs round-robin (block b runs
