@@ -5,9 +5,10 @@
 //
 // Plain sweep, vector form: each lane owns V consecutive j (one 16-byte
 // load: 4 FP32, 2 FP64, 8 BF16); a 256-thread workgroup is 64 lanes x 4 row
-// strips and each thread marches down kRowsVec rows keeping the rows
-// above/at/below its points in registers; the loads of kUnroll rows are
-// issued before any is used.  The j+-1 edge scalars are L1 hits; tiles are
+// strips and each thread marches down kRowsVec rows (16, or 4 when 16 would
+// leave the device with fewer than four workgroups per CU, as a 1000^2 FP64
+// grid would) keeping the rows above/at/below its points in registers; the
+// loads of kUnroll rows are issued before any is used.  The j+-1 edge scalars are L1 hits; tiles are
 // numbered XCD by XCD so neighbouring strips share an L2.  Every src element
 // crosses HBM about once per sweep.
 //
@@ -49,10 +50,9 @@ __global__ __launch_bounds__(kLanes* kStrips) void jacobi2d_sweep(const S* __res
   }
 }
 
-constexpr int kRowsVec = 16;
 constexpr int kUnroll = 4;
 
-template <class S>
+template <class S, int kRowsVec>
 __global__ __launch_bounds__(kLanes* kStrips) void jacobi2d_sweep_vec(const S* __restrict__ src,
                                                                      S* __restrict__ dst, int n,
                                                                      int tiles_j) {
@@ -114,7 +114,10 @@ int run_sweeps(S* A, S* B, int n, int64_t first, int64_t nsweeps, hipStream_t st
   const dim3 block(kLanes, kStrips);
   const bool vec = n % V == 0 && aligned16(A) && aligned16(B);
   const int tiles_j = static_cast<int>(cdiv(n, kLanes * V));
-  const int tiles_i = static_cast<int>(cdiv(n - 2, kStrips * kRowsVec));
+  // rows per thread: 16, or 4 when 16 gives fewer than ~4 workgroups per CU
+  const bool short_rows = int64_t(tiles_j) * cdiv(n - 2, kStrips * 16) < 1024;
+  const int rows = short_rows ? 4 : 16;
+  const int tiles_i = static_cast<int>(cdiv(n - 2, kStrips * rows));
   const unsigned nblocks = static_cast<unsigned>(int64_t(tiles_j) * tiles_i);
   const dim3 grid(static_cast<unsigned>(cdiv(n - 2, kLanes)),
                   static_cast<unsigned>(cdiv(n - 2, kStrips * kRows)));
@@ -122,9 +125,13 @@ int run_sweeps(S* A, S* B, int n, int64_t first, int64_t nsweeps, hipStream_t st
     const bool even = (s % 2) == 0;
     const S* src = even ? A : B;
     S* dst = even ? B : A;
-    if (vec) {
-      hipLaunchKernelGGL(jacobi2d_sweep_vec<S>, dim3(nblocks), block, 0, stream, src, dst, n,
-                         tiles_j);
+    if (vec && short_rows) {
+      hipLaunchKernelGGL((jacobi2d_sweep_vec<S, 4>), dim3(nblocks), block, 0, stream, src, dst,
+                         n, tiles_j);
+      NPBX_LAUNCHED("jacobi2d_sweep_vec");
+    } else if (vec) {
+      hipLaunchKernelGGL((jacobi2d_sweep_vec<S, 16>), dim3(nblocks), block, 0, stream, src, dst,
+                         n, tiles_j);
       NPBX_LAUNCHED("jacobi2d_sweep_vec");
     } else {
       hipLaunchKernelGGL(jacobi2d_sweep<S>, grid, block, 0, stream, src, dst, n);
