@@ -169,16 +169,24 @@ def single_grid(st, dev, torch, clock, args, A64, dtype, shape):
 
 
 def fp32_error(st, dev, torch, A64, B64):
+    """FP32 storage against the FP64 GPU path (bitwise the NumPy oracle) on
+    the headline configuration: the inf-norm relative error, the elementwise
+    maximum of |x - x64| / max(|x64|, 1e-3 max|x64|), and the reference
+    harness's 2-norm ratio (utilities.py:27-28)."""
     a32, b32 = dev.to_device(A64, torch.float32), dev.to_device(B64, torch.float32)
     a64, b64 = dev.to_device(A64), dev.to_device(B64)
     st.heat3d_sweeps(a32, b32, SWEEPS)
     st.heat3d_sweeps(a64, b64, SWEEPS)
     torch.cuda.synchronize()
-    return (a32.double() - a64).abs().max().item() / a64.abs().max().item()
+    d = (a32.double() - a64).abs()
+    scale = a64.abs().max()
+    return {"inf_norm": d.max().item() / scale.item(),
+            "elementwise": (d / torch.clamp(a64.abs(), min=1e-3 * scale)).max().item(),
+            "norm2": (torch.linalg.vector_norm(d) / torch.linalg.vector_norm(a64)).item()}
 
 
 # ----------------------------------------------------------------- serving
-def serving(torch, args, heat_rows=SERVE_HEAT_ROWS):
+def serving(torch, args, heat_rows=SERVE_HEAT_ROWS, cpu_rates=None):
     from npbench_amd import _native as nat, serve
     nat.set_tuning("heat.rows", heat_rows)
     reqs = serve.poisson_mix(args.serve_requests, args.serve_qps, seed=42)
@@ -192,7 +200,32 @@ def serving(torch, args, heat_rows=SERVE_HEAT_ROWS):
                 "side U[32,256], steps U[10,200], FP32, seed 42", "tblock": 2,
                 "heat_rows": heat_rows, "peak_active": srv.peak_active, "ticks": srv.ticks})
     srv.close()
+    if cpu_rates:
+        out["cpu_reference"] = cpu_serving_reference(reqs, cpu_rates)
     return out
+
+
+def cpu_serving_reference(reqs, rates):
+    """The same arrivals served one at a time by the NumPy oracle on one host
+    core: each request's service time is its LUPs at the oracle rate measured
+    for its kind (GLUP/s, bounded samples), queued first come first served."""
+    free_at = 0.0
+    lat, service = [], []
+    for r in sorted(reqs, key=lambda q: q.arrival):
+        svc = r.lups / (rates[r.kind] * 1e9)
+        start = max(free_at, r.arrival)
+        free_at = start + svc
+        lat.append(free_at - r.arrival)
+        service.append(svc)
+    lat = np.array(lat) * 1e3
+    svc = np.array(service) * 1e3
+    return {"service_p50_ms": round(float(np.percentile(svc, 50)), 1),
+            "service_p99_ms": round(float(np.percentile(svc, 99)), 1),
+            "queued_p50_ms": round(float(np.percentile(lat, 50)), 1),
+            "queued_p99_ms": round(float(np.percentile(lat, 99)), 1),
+            "rates_glups": {k: round(v, 4) for k, v in rates.items()}, "cores": 1,
+            "note": "service time = request LUPs / oracle GLUP/s of its kind (sampled on this "
+                    "host); latencies from a single-server FCFS queue at the same arrivals"}
 
 
 # ----------------------------------------------------------------- kernels
@@ -245,6 +278,17 @@ def kernel_suite(st, dev, torch, clock, reps, copy_only=False):
         res[f"hdiff_256x256x160_{tag}"] = {
             "glups": round(I * J * K * reps / ev / 1e9, 2), "avg_us": round(ev / reps * 1e6, 3),
             "hbm_frac": round(byts * reps / ev / 1e9 / HBM_PEAK_GBS, 4)}
+        if tag == "f32":
+            # iterated form (one launch per step through a workspace field)
+            for steps in (4, 8):
+                def hdn(steps=steps):
+                    st.hdiff(di, do, dc, nsteps=steps, stream=clock.stream)
+                hdn()
+                _, ev = clock.run(hdn, reps)
+                res[f"hdiff_256x256x160_f32_iterated{steps}"] = {
+                    "glups": round(I * J * K * steps * reps / ev / 1e9, 2),
+                    "avg_step_us": round(ev / reps / steps * 1e6, 3),
+                    "hbm_frac": round(byts * steps * reps / ev / 1e9 / HBM_PEAK_GBS, 4)}
         del di, dc, do
 
     fields = [rng.random((I, J, K)), rng.random((I, J, K)), 2 * rng.random((I + 1, J, K)) - 1,
@@ -262,6 +306,22 @@ def kernel_suite(st, dev, torch, clock, reps, copy_only=False):
             "hbm_frac": round(byts * reps / ev / 1e9 / HBM_PEAK_GBS, 4)}
         del d
     return res
+
+
+def cpu_rates():
+    """Oracle GLUP/s per kind on bounded samples (1 core): Jacobi 1024^2 x 10
+    sweeps and heat 128^3 x 4 sweeps, FP64, the reference NumPy path."""
+    from oracle import stencils as ref
+    rng = np.random.default_rng(1)
+    out = {}
+    for kind, n, sweeps in (("jacobi_2d", 1024, 10), ("heat_3d", 128, 4)):
+        shape = (n, n) if kind == "jacobi_2d" else (n, n, n)
+        A, B = rng.random(shape), rng.random(shape)
+        f = ref.jacobi_sweeps if kind == "jacobi_2d" else ref.heat_sweeps
+        t0 = time.perf_counter()
+        f(A, B, sweeps)
+        out[kind] = (n - 2) ** len(shape) * sweeps / (time.perf_counter() - t0) / 1e9
+    return out
 
 
 def cpu_baseline(n, sweeps):
@@ -370,10 +430,12 @@ def main():
             "value": round(f32["glups"], 3), "unit": "GLUP/s",
             "roofline": roofline(8.0 * lup_sweep, f32["launch_s"], "heat3d_sweep_vec<float>")},
         "speedup_vs_fp64_plain": round(head["glups"] / f64["glups"], 3),
-        "max_rel_err_vs_fp64": err,
+        "max_rel_err_vs_fp64": err["inf_norm"],
+        "fp32_error_vs_fp64": err,
     }
     if not args.no_serve:
-        line["serving"] = serving(torch, args)
+        line["serving"] = serving(torch, args,
+                                  cpu_rates=cpu_rates() if args.cpu_sweeps > 0 else None)
     if not args.no_kernels:
         line["kernels"] = kernel_suite(st, dev, torch, clock, 5)
     if args.cpu_sweeps > 0:
