@@ -278,17 +278,24 @@ def kernel_suite(st, dev, torch, clock, reps, copy_only=False):
         res[f"hdiff_256x256x160_{tag}"] = {
             "glups": round(I * J * K * reps / ev / 1e9, 2), "avg_us": round(ev / reps * 1e6, 3),
             "hbm_frac": round(byts * reps / ev / 1e9 / HBM_PEAK_GBS, 4)}
-        if tag == "f32":
-            # iterated form (one launch per step through a workspace field)
+        if tag in ("f32", "bf16"):
+            # iterated form: one launch per step (tblock 1) or tblock steps
+            # fused per HBM round trip; hbm_frac counts one step's bytes per
+            # step, so it exceeds the roofline's meaning once steps are fused
+            tbs = (1, 2, 4) if tag == "f32" else (1, 2)
             for steps in (4, 8):
-                def hdn(steps=steps):
-                    st.hdiff(di, do, dc, nsteps=steps, stream=clock.stream)
-                hdn()
-                _, ev = clock.run(hdn, reps)
-                res[f"hdiff_256x256x160_f32_iterated{steps}"] = {
-                    "glups": round(I * J * K * steps * reps / ev / 1e9, 2),
-                    "avg_step_us": round(ev / reps / steps * 1e6, 3),
-                    "hbm_frac": round(byts * steps * reps / ev / 1e9 / HBM_PEAK_GBS, 4)}
+                for tb in tbs:
+                    nat.set_tuning("hdiff.tblock", tb)
+
+                    def hdn(steps=steps):
+                        st.hdiff(di, do, dc, nsteps=steps, stream=clock.stream)
+                    hdn()
+                    _, ev = clock.run(hdn, reps)
+                    res[f"hdiff_256x256x160_{tag}_steps{steps}_tblock{tb}"] = {
+                        "glups": round(I * J * K * steps * reps / ev / 1e9, 2),
+                        "avg_step_us": round(ev / reps / steps * 1e6, 3),
+                        "effective_frac": round(byts * steps * reps / ev / 1e9 / HBM_PEAK_GBS, 4)}
+            nat.set_tuning("hdiff.tblock", 1)
         del di, dc, do
 
     fields = [rng.random((I, J, K)), rng.random((I, J, K)), 2 * rng.random((I + 1, J, K)) - 1,

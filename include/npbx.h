@@ -194,7 +194,8 @@ int npbx_pack_cols(int dtype, const void *A, const void *B, int64_t n,
  * wave-private heat tile: rows per wave including its own 2T halo rows),
  * "heat.seg" = default planes per heat tile, "sweep.shape" = the plain
  * sweep's planes per workgroup / planes loaded ahead (0: 16/4, 1: 16/8,
- * 2: 32/8, 3: 8/4).  Results do not depend on them. */
+ * 2: 32/8, 3: 8/4), "hdiff.tblock" = iterated hdiff steps fused per HBM round
+ * trip (1, the default, 2 or 4; BF16 at most 2).  Results do not depend on them. */
 int npbx_set_tuning(const char *key, int64_t value);
 int npbx_get_tuning(const char *key, int64_t *value);
 

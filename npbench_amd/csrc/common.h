@@ -121,6 +121,9 @@ int tiled_sweeps(int kind, int dtype, void* A, void* B, int n, int64_t first, in
 // per workgroup with loads of 4 planes in flight, 1 = 16/8, 2 = 32/8,
 // 3 = 8/4).
 int sweep_shape();
+// Fused hdiff steps per round ("hdiff.tblock"; 1 = one launch per step).
+int hdiff_tblock();
+void set_hdiff_tblock(int tb);
 
 // Per-device scratch owned by the library (grow-only, freed at exit).  Used
 // by kernels that need storage beyond their inputs and outputs.

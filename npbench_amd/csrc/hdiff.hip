@@ -28,6 +28,7 @@
 // the original in_field held fixed.  Steps ping-pong through a library
 // workspace holding one (I+4, J+4, K) field; in_field is never written.
 #include <algorithm>
+#include <atomic>
 
 #include "common.h"
 #include "vecio.h"
@@ -239,6 +240,134 @@ __global__ __launch_bounds__(256) void hdiff_march(const S* __restrict__ in, S* 
   }
 }
 
+// ---------------------------------------------------------------- fused steps
+// hdiff_tb<S, TB>: up to TB iterated steps in one HBM round trip.  A
+// workgroup owns a 32 x 32 block of output points (i, j) for one k-vector
+// (V = 16 bytes of k) and stages the input field over the block plus a halo
+// of 2 cells per fused step in LDS; each step recomputes the region shrunk
+// by 2 on every side into the other LDS buffer (field-halo points keep their
+// fixed values; intermediate values are rounded through the storage type as
+// the iterated form's stores round them), and the last step writes only the
+// block.  Per point the arithmetic is hdiff_point's, so results are bitwise
+// the iterated form's.  Consecutive workgroups are the k-vectors of one
+// block (XCD-contiguous), so the 128-byte lines a workgroup reads 16 bytes
+// of are read by its neighbours from the same L2.  It cuts HBM bytes per
+// step (32x32 blocks, two steps: about 0.6x) but each wave load touches one
+// line per lane, and measured slower than one launch per step.
+constexpr int kTbB = 32;  // output block side (i and j)
+
+template <class S, int TB>
+struct TbShape {
+  static constexpr int R = kTbB + 4 * TB;  // side of the staged region
+  static constexpr int V = Vec16<S>::N;
+  static constexpr int ELEMS = R * R * V;  // one LDS field buffer, arithmetic type
+};
+
+template <class S, int TB>
+__global__ __launch_bounds__(256) void hdiff_tb(const S* __restrict__ src,
+                                                const S* __restrict__ coeff, S* __restrict__ out,
+                                                S* __restrict__ next, int I, int J, int K, int steps,
+                                                int tiles_y, int nblocks) {
+  using St = Storage<S>;
+  using C = typename St::C;
+  using Sh = TbShape<S, TB>;
+  constexpr int V = Sh::V, R = Sh::R;
+  __shared__ __align__(16) C fa[Sh::ELEMS];
+  __shared__ __align__(16) C fb[Sh::ELEMS];
+  __shared__ __align__(16) C cfs[Sh::ELEMS];
+  const int kvs = K / V;
+  const int b = xcd_tile(blockIdx.x, nblocks);
+  const int kc = b % kvs, tile = b / kvs;
+  const int X0 = (tile / tiles_y) * kTbB, Y0 = (tile % tiles_y) * kTbB;
+  const int pa = X0 + 2 - 2 * TB, qa = Y0 + 2 - 2 * TB;  // field point of buffer (0, 0)
+  const int64_t si = static_cast<int64_t>(J + 4) * K, sj = K;
+  const int k0 = kc * V;
+  auto in_field = [&](int p, int q) { return p >= 0 && p < I + 4 && q >= 0 && q < J + 4; };
+  auto interior = [&](int p, int q) { return p >= 2 && p < I + 2 && q >= 2 && q < J + 2; };
+
+  for (int e = threadIdx.x; e < R * R; e += 256) {
+    const int i = e / R, j = e % R, p = pa + i, q = qa + j;
+    C v[V];
+    if (in_field(p, q)) {
+      load16(src + p * si + q * sj + k0, v);
+    } else {
+#pragma unroll
+      for (int u = 0; u < V; ++u) v[u] = C(0);
+    }
+#pragma unroll
+    for (int u = 0; u < V; ++u) fa[e * V + u] = v[u];
+    if (interior(p, q) && i >= 2 && i < R - 2 && j >= 2 && j < R - 2) {
+      load16(coeff + ((static_cast<int64_t>(p) - 2) * J + (q - 2)) * K + k0, v);
+#pragma unroll
+      for (int u = 0; u < V; ++u) cfs[e * V + u] = v[u];
+    }
+  }
+  __syncthreads();
+
+  // One step from buffer A into buffer B over region [lo, R-lo)^2 (t < steps)
+  // or into global memory over the output block (t == steps).
+  auto step = [&](const C* __restrict__ A, C* __restrict__ B, int t) {
+    const bool last = t == steps;
+    const int lo = last ? 2 * TB : 2 * t, n = last ? kTbB : R - 4 * t;
+    for (int e = threadIdx.x; e < n * n; e += 256) {
+      const int i = lo + e / n, j = lo + e % n, p = pa + i, q = qa + j;
+      if (!in_field(p, q)) continue;
+      const int at = i * R + j;
+      C res[V];
+      if (!interior(p, q)) {  // the field's halo: fixed
+        if (last) continue;
+#pragma unroll
+        for (int u = 0; u < V; ++u) res[u] = A[at * V + u];
+      } else {
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+          auto F = [&](int idx) { return A[idx * V + u]; };
+          auto lap = [&](int ci) {
+            C nb = F(ci + R) + F(ci - R);
+            nb = nb + F(ci + 1);
+            nb = nb + F(ci - 1);
+            return static_cast<C>(4.0) * F(ci) - nb;
+          };
+          const C c = F(at), ip = F(at + R), im = F(at - R), jp = F(at + 1), jm = F(at - 1);
+          const C l0 = lap(at);
+          const C flx_hi = limit<C>(lap(at + R) - l0, ip - c);
+          const C flx_lo = limit<C>(l0 - lap(at - R), c - im);
+          const C fly_hi = limit<C>(lap(at + 1) - l0, jp - c);
+          const C fly_lo = limit<C>(l0 - lap(at - 1), c - jm);
+          C d = flx_hi - flx_lo;
+          d = d + fly_hi;
+          d = d - fly_lo;
+          res[u] = St::load(St::store(c - cfs[at * V + u] * d));
+        }
+      }
+      if (!last) {
+#pragma unroll
+        for (int u = 0; u < V; ++u) B[at * V + u] = res[u];
+      } else if (p - 2 < I && q - 2 < J) {
+        if (next)
+          store16(next + p * si + q * sj + k0, res);
+        else
+          store16(out + ((static_cast<int64_t>(p) - 2) * J + (q - 2)) * K + k0, res);
+      }
+    }
+  };
+  for (int t = 1; t <= steps; ++t) {
+    if (t & 1)
+      step(fa, fb, t);
+    else
+      step(fb, fa, t);
+    __syncthreads();
+  }
+}
+
+// Fused steps per round (tuning knob "hdiff.tblock"; 1 = one launch per step,
+// the default: at 256x256x160 on the MI355X the fused kernel is slower, its
+// lanes each reading 16 bytes of a different 640-byte k-column).
+std::atomic<int> g_hdiff_tblock{1};
+
+template <class S>
+constexpr int hdiff_tb_max() { return sizeof(S) == 2 ? 2 : 4; }  // BF16 x4 exceeds LDS
+
 template <class S>
 int run_hdiff(const S* in, S* out, const S* coeff, int I, int J, int K, int64_t nsteps,
               hipStream_t stream) {
@@ -264,6 +393,7 @@ int run_hdiff(const S* in, S* out, const S* coeff, int I, int J, int K, int64_t 
     return NPBX_OK;
   };
   if (nsteps == 1) return launch(in, nullptr);
+  const int tb = std::min(hdiff_tblock(), hdiff_tb_max<S>());
   // Two work fields ping-pong; each carries the original halo.
   const size_t field = static_cast<size_t>(I + 4) * (J + 4) * K;
   void* ws = nullptr;
@@ -276,6 +406,32 @@ int run_hdiff(const S* in, S* out, const S* coeff, int I, int J, int K, int64_t 
     NPBX_LAUNCHED("hdiff_copy_halo");
   }
   const S* src = in;
+  if (tb >= 2 && vec) {
+    constexpr int V = Vec16<S>::N;
+    const int tiles_x = static_cast<int>(cdiv(I, kTbB)), tiles_y = static_cast<int>(cdiv(J, kTbB));
+    const int64_t nb = int64_t(tiles_x) * tiles_y * (K / V);
+    NPBX_REQUIRE(nb < (int64_t(1) << 31), "hdiff: grid too large");
+    int64_t done = 0, round = 0;
+    while (done < nsteps) {
+      const int steps = static_cast<int>(std::min<int64_t>(tb, nsteps - done));
+      const bool last = done + steps == nsteps;
+      S* next = last ? nullptr : w[round % 2];
+      auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(nb)), dim3(256), 0, stream, src,
+                           coeff, out, next, I, J, K, steps, tiles_y, static_cast<int>(nb));
+      };
+      if (tb >= 4) {
+        if constexpr (hdiff_tb_max<S>() >= 4) go(hdiff_tb<S, 4>);
+      } else {
+        go(hdiff_tb<S, 2>);
+      }
+      NPBX_LAUNCHED("hdiff_tb");
+      src = next;
+      done += steps;
+      ++round;
+    }
+    return NPBX_OK;
+  }
   for (int64_t n = 0; n < nsteps; ++n) {
     const bool last = n + 1 == nsteps;
     S* next = w[n % 2];
@@ -286,6 +442,10 @@ int run_hdiff(const S* in, S* out, const S* coeff, int I, int J, int K, int64_t 
 }
 
 }  // namespace
+
+int hdiff_tblock() { return g_hdiff_tblock.load(); }
+void set_hdiff_tblock(int tb) { g_hdiff_tblock.store(tb); }
+
 }  // namespace npbx
 
 extern "C" int npbx_hdiff(int dtype, const void* in_field, void* out_field, const void* coeff,

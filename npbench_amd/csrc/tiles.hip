@@ -1345,8 +1345,14 @@ extern "C" int npbx_set_tuning(const char* key, int64_t value) {
     g_sweep_shape.store(static_cast<int>(value));
     return NPBX_OK;
   }
-  return fail(NPBX_EINVAL, "set_tuning: unknown key '%s' (heat.rows, heat.seg, sweep.shape)",
-              key);
+  if (std::strcmp(key, "hdiff.tblock") == 0) {
+    NPBX_REQUIRE(value == 1 || value == 2 || value == 4,
+                 "set_tuning: hdiff.tblock must be 1, 2 or 4");
+    set_hdiff_tblock(static_cast<int>(value));
+    return NPBX_OK;
+  }
+  return fail(NPBX_EINVAL,
+              "set_tuning: unknown key '%s' (heat.rows, heat.seg, sweep.shape, hdiff.tblock)", key);
 }
 
 extern "C" int npbx_get_tuning(const char* key, int64_t* value) {
@@ -1361,6 +1367,10 @@ extern "C" int npbx_get_tuning(const char* key, int64_t* value) {
   }
   if (std::strcmp(key, "sweep.shape") == 0) {
     *value = g_sweep_shape.load();
+    return NPBX_OK;
+  }
+  if (std::strcmp(key, "hdiff.tblock") == 0) {
+    *value = hdiff_tblock();
     return NPBX_OK;
   }
   return fail(NPBX_EINVAL, "get_tuning: unknown key '%s'", key);

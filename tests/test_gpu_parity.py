@@ -293,3 +293,41 @@ def test_cast_bf16_rne(st):
     b32 = st.cast(dev(x32), torch.bfloat16)
     want = torch.from_numpy(x32).to(torch.bfloat16)
     assert torch.equal(b32.cpu().view(torch.int16), want.view(torch.int16))
+
+
+@pytest.fixture
+def hdiff_tblock():
+    """Sets the library's hdiff.tblock knob for one test and restores it."""
+    from npbench_amd import _native
+    old = _native.get_tuning("hdiff.tblock")
+    yield lambda tb: _native.set_tuning("hdiff.tblock", tb)
+    _native.set_tuning("hdiff.tblock", old)
+
+
+@pytest.mark.parametrize("precision,tb", [("fp64", 2), ("fp64", 4), ("fp32", 2), ("fp32", 4),
+                                          ("bf16", 2)])
+@pytest.mark.parametrize("I,J,K", [(1, 1, 8), (20, 24, 40), (70, 33, 16), (64, 96, 24)])
+@pytest.mark.parametrize("nsteps", [2, 3, 5, 8])
+def test_hdiff_fused_matches_iterated(st, hdiff_tblock, precision, tb, I, J, K, nsteps):
+    """Fused steps (hdiff.tblock 2 / 4, 32 x 32 blocks staged in LDS) give
+    bitwise the one-launch-per-step iteration, for blocks cut by the grid
+    edge, rounds shorter than tblock and every storage type; FP64 is also
+    bitwise the oracle's iteration."""
+    print(f"hdiff fused {precision} tb={tb} {I}x{J}x{K} nsteps={nsteps}", flush=True)
+    dt = {"fp64": torch.float64, "fp32": torch.float32, "bf16": torch.bfloat16}[precision]
+    inf, out, coeff = _hdiff_inputs(I, J, K, I + J + K + nsteps)
+    d_in, d_c = dev(inf, dt), dev(coeff, dt)
+    hdiff_tblock(1)
+    it = dev(out, dt)
+    st.hdiff(d_in, it, d_c, nsteps=nsteps)
+    hdiff_tblock(tb)
+    fu = dev(out, dt)
+    st.hdiff(d_in, fu, d_c, nsteps=nsteps)
+    torch.cuda.synchronize()
+    eq = torch.equal(fu.view(torch.int16), it.view(torch.int16)) if dt == torch.bfloat16 \
+        else torch.equal(fu, it)
+    assert eq, "fused != iterated"
+    if precision == "fp64":
+        r = out.copy()
+        ref.hdiff_iterated(inf, r, coeff, nsteps)
+        assert_bitwise(host(fu), r, "out_field")
