@@ -41,7 +41,11 @@ TSTEPS = 101
 SWEEPS = 2 * (TSTEPS - 1)
 BATCH = 8
 SIZE = {"f64": 8, "f32": 4, "bf16": 2}
-SERVE_HEAT_ROWS = 12  # serving's heat tiles: the wave-private FP32 tile
+SERVE_HEAT_ROWS = 10  # serving's heat tiles: the wave-private FP32 tile (10 rows
+                      # measured lowest p50/p99 on the serving mix)
+SERVE_TBLOCK = 2
+SERVE_SEG = 32  # heat planes per tile for serving (set explicitly: the process-wide
+                # knob holds whatever the headline's tuner chose)
 
 
 def cpu_model():
@@ -186,19 +190,23 @@ def fp32_error(st, dev, torch, A64, B64):
 
 
 # ----------------------------------------------------------------- serving
-def serving(torch, args, heat_rows=SERVE_HEAT_ROWS, cpu_rates=None):
+def serving(torch, args, heat_rows=SERVE_HEAT_ROWS, cpu_rates=None, tblock=SERVE_TBLOCK):
     from npbench_amd import _native as nat, serve
     nat.set_tuning("heat.rows", heat_rows)
     reqs = serve.poisson_mix(args.serve_requests, args.serve_qps, seed=42)
-    srv = serve.Server(dtype=torch.float32, tblock=2, hbm_budget=32 << 30, max_active=48)
-    srv.run(serve.poisson_mix(8, 1e6, seed=7), realtime=False)  # warm-up
+    srv = serve.Server(dtype=torch.float32, tblock=tblock, hbm_budget=32 << 30, max_active=48,
+                       seg=SERVE_SEG)
+    # warm-up: kernels' first launches and the allocator's pool for a full mix
+    srv.run(serve.poisson_mix(64, 1e6, seed=7), realtime=False)
+    srv.peak_active = srv.peak_used = srv.ticks = 0
     t0 = time.perf_counter()
     done = srv.run(reqs, realtime=True)
     wall = time.perf_counter() - t0
     out = serve.summarize(done, wall)
     out.update({"qps": args.serve_qps, "mix": "50% jacobi_2d side log-U[128,4096], 50% heat_3d "
-                "side U[32,256], steps U[10,200], FP32, seed 42", "tblock": 2,
-                "heat_rows": heat_rows, "peak_active": srv.peak_active, "ticks": srv.ticks})
+                "side U[32,256], steps U[10,200], FP32, seed 42", "tblock": tblock,
+                "heat_rows": heat_rows, "heat_seg": SERVE_SEG, "peak_active": srv.peak_active,
+                "ticks": srv.ticks})
     srv.close()
     if cpu_rates:
         out["cpu_reference"] = cpu_serving_reference(reqs, cpu_rates)
