@@ -63,6 +63,10 @@ int npbx_last_error(char *buf, size_t len);
 int npbx_device_count(int *count);
 /* Selects the device for this thread and checks it is a gfx950 part. */
 int npbx_init(int device);
+/* Releases the library-owned scratch of the current device (waits for the
+ * device first); later calls allocate it again.  Replaces nothing in the
+ * reference (host-side teardown). */
+int npbx_finalize(void);
 
 /* ---- memory, streams, events (framework.py:42-50 copy_func/copy_back_func) */
 int npbx_malloc(void **ptr, size_t bytes);

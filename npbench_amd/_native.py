@@ -85,6 +85,7 @@ SIGNATURES = {
                            ctypes.c_int32, _vp, _vp],
     "npbx_cols_elems": [_i64, ctypes.POINTER(_i64)],
     "npbx_pack_cols": [_int, _vp, _vp, _i64, _vp, _vp],
+    "npbx_finalize": [],
     "npbx_set_tuning": [ctypes.c_char_p, _i64],
     "npbx_get_tuning": [ctypes.c_char_p, ctypes.POINTER(_i64)],
     "npbx_hdiff": [_int, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _vp],

@@ -133,6 +133,24 @@ int npbx_init(int device) {
   return NPBX_OK;
 }
 
+// Releases the library-owned workspace of the current device (after the
+// device has drained); the next call that needs it allocates again.  Plans
+// and queues are released by their own destroy calls.
+int npbx_finalize(void) {
+  int dev = 0;
+  NPBX_HIP(hipGetDevice(&dev));
+  NPBX_REQUIRE(dev >= 0 && dev < npbx::kMaxDevices, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lock(npbx::g_workspace_mu);
+  npbx::Workspace& ws = npbx::g_workspace[dev];
+  if (ws.ptr) {
+    NPBX_HIP(hipDeviceSynchronize());
+    NPBX_HIP(hipFree(ws.ptr));
+    ws.ptr = nullptr;
+    ws.bytes = 0;
+  }
+  return NPBX_OK;
+}
+
 int npbx_malloc(void** ptr, size_t bytes) {
   NPBX_REQUIRE(ptr, "npbx_malloc: null pointer");
   *ptr = nullptr;

@@ -278,3 +278,27 @@ def test_speedup_report_from_the_result_store(tmp_path):
     pytest.importorskip("matplotlib")
     report.heatmap(sp, str(tmp_path / "s.png"))
     assert (tmp_path / "s.png").stat().st_size > 0
+
+
+def test_run_benchmark_cli(tmp_path):
+    """The run_benchmark.py counterpart: options, a NumPy run of the S
+    preset, result rows in the database and the JSON-lines file."""
+    import json
+    import sqlite3
+    from npbench_amd import run_benchmark
+    a = run_benchmark.parse(["-b", "jacobi_2d"])
+    assert (a["framework"], a["preset"], a["validate"], a["repeat"], a["timeout"]) == \
+        ("numpy", "S", True, 10, 200.0)
+    with pytest.raises(SystemExit):
+        run_benchmark.parse(["-b", "gemm"])  # only the stencil path is built
+    db, jl = tmp_path / "r.db", tmp_path / "r.jsonl"
+    rc = run_benchmark.main(["-b", "jacobi_2d", "-f", "numpy", "-r", "2", "-v", "false",
+                             "--database", str(db), "--jsonl", str(jl)])
+    assert rc == 0
+    con = sqlite3.connect(db)
+    n = con.execute("select count(*) from results where benchmark='jacobi2d' "
+                    "and framework='numpy'").fetchone()[0]
+    con.close()
+    assert n == 2
+    rows = [json.loads(x) for x in jl.read_text().splitlines()]
+    assert len(rows) == 2 and all(r["glups"] > 0 for r in rows)
