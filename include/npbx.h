@@ -183,7 +183,8 @@ int npbx_rounds_launch(const npbx_round_spec *rounds, int32_t count,
                        int32_t tblock, int32_t seg, void *queue, void *stream);
 
 /* Boundary columns of an FP32 heat grid (columns 0 and n-1 of every row of A
- * and of B), packed into `cols` of npbx_cols_elems(n) = 4*n*n floats, for
+ * and of B), packed into `cols` of npbx_cols_elems(n) = 4*n*n + 1 floats
+ * (the last word records whether A's and B's boundaries are equal), for
  * npbx_round_spec.cols.  Pack again after A's or B's boundary changes.  No
  * counterpart in the reference (a layout for the batched/serving forms). */
 int npbx_cols_elems(int64_t n, int64_t *elems);

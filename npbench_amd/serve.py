@@ -61,7 +61,7 @@ class Request:
 
     def footprint(self, dtype):
         """Grid bytes plus, for FP32 heat, the packed boundary columns."""
-        cols = 16 * self.n ** 2 if (self.kind == "heat_3d" and dtype == torch.float32) else 0
+        cols = 16 * self.n ** 2 + 4 if (self.kind == "heat_3d" and dtype == torch.float32) else 0
         return 2 * self.n ** self.ndim * ELEM[dtype] + cols
 
 

@@ -34,6 +34,29 @@ def grids(kind, n, dtype, seed):
     return A, B
 
 
+def share_boundary(A, B):
+    """B with A's global-boundary values (all faces), its interior kept."""
+    B = B.copy()
+    inner = tuple(slice(1, -1) for _ in A.shape)
+    mask = np.ones(A.shape, bool)
+    mask[inner] = False
+    B[mask] = A[mask]
+    return B
+
+
+# one point of each face of a heat grid: (axis, side)
+FACES = [(0, 0), (0, -1), (1, 0), (1, -1), (2, 0), (2, -1)]
+
+
+def face_point(n, face, seed):
+    """An index of a point on `face` away from its edges (a 3-D grid)."""
+    axis, side = face
+    rng = np.random.default_rng(seed)
+    idx = [int(x) for x in rng.integers(1, n - 1, size=3)]
+    idx[axis] = 0 if side == 0 else n - 1
+    return tuple(idx)
+
+
 def run(st, kind, A, B, dtype, nsweeps, first, tblock):
     from npbench_amd import device
     a, b = device.to_device(A, DT[dtype]), device.to_device(B, DT[dtype])
@@ -100,9 +123,11 @@ def _mixed_batch(dtype, seed):
              ("jacobi_2d", 1024, 3, 0), ("heat_3d", 132, 5, 1)]
     out = []
     vec = {"f64": 2, "f32": 4, "bf16": 8}[dtype]
-    for kind, n, ns, first in specs:
+    for i, (kind, n, ns, first) in enumerate(specs):
         n = -(-n // vec) * vec  # whole 16-byte vectors per row
         A, B = grids(kind, n, dtype, int(rng.integers(1 << 30)))
+        if i % 4 == 3:  # heat grids of equal boundaries (as B = A.copy() gives)
+            B = share_boundary(A, B)
         out.append((kind, A, B, ns, first))
     return out
 
@@ -191,17 +216,39 @@ WAVE_SHAPES = [(2, 10), (2, 12), (3, 12), (3, 14)]  # (T, rows)
 @pytest.mark.parametrize("T,rows", WAVE_SHAPES)
 @pytest.mark.parametrize("n", WAVE_HEAT_CASES)
 @pytest.mark.parametrize("nsweeps,first", [(1, 1), (2, 0), (3, 0), (6, 1), (11, 0)])
-def test_heat_wave_tiles_match_plain(st, heat_rows, T, rows, n, nsweeps, first):
+@pytest.mark.parametrize("bnd", ["different", "equal"])
+def test_heat_wave_tiles_match_plain(st, heat_rows, T, rows, n, nsweeps, first, bnd):
     """The wave-private FP32 heat tile (heat.rows 10 .. 14, depth 2 and 3):
     every size class -- grids of a few rows (several boundary rows in one
     wave), one and several tiles along j, one and several along k, first and
     last planes, rounds of every depth -- with A and B of different
-    boundaries, bitwise the plain sweeps."""
+    boundaries (boundary values patched in from the packed columns and the
+    arrays) and of equal ones (taken from registers), bitwise the plain
+    sweeps."""
     A, B = grids("heat_3d", n, "f32", 7 * n + nsweeps)
+    if bnd == "equal":
+        B = share_boundary(A, B)
     pa, pb = run(st, "heat_3d", A, B, "f32", nsweeps, first, 1)
     heat_rows(rows)
-    print(f"heat wave tile T={T} rows={rows} n={n} sweeps={nsweeps} first={first}", flush=True)
+    print(f"heat wave tile T={T} rows={rows} n={n} sweeps={nsweeps} first={first} {bnd}",
+          flush=True)
     ta, tb = run(st, "heat_3d", A, B, "f32", nsweeps, first, T)
+    assert same(ta, pa) and same(tb, pb)
+
+
+@pytest.mark.parametrize("T,rows", [(2, 12), (3, 14)])
+@pytest.mark.parametrize("n", [20, 68])
+@pytest.mark.parametrize("face", FACES)
+def test_heat_wave_tiles_one_boundary_point_differs(st, heat_rows, T, rows, n, face):
+    """Boundaries equal but for one point of one face: the grid must take the
+    patching tile, which reproduces that point's level-parity value."""
+    A, B = grids("heat_3d", n, "f32", 11 * n)
+    B = share_boundary(A, B)
+    B[face_point(n, face, n)] += 0.5
+    pa, pb = run(st, "heat_3d", A, B, "f32", 6, 0, 1)
+    heat_rows(rows)
+    print(f"heat wave tile T={T} rows={rows} n={n} face={face}", flush=True)
+    ta, tb = run(st, "heat_3d", A, B, "f32", 6, 0, T)
     assert same(ta, pa) and same(tb, pb)
 
 
@@ -226,7 +273,9 @@ def test_plan_batch_wave_tiles(st, heat_rows, T, rows):
 
 
 def test_pack_cols_layout(st):
-    """npbx_pack_cols: cols[a][q][j][s] = (A, B)[a][q][j][(0, n-1)[s]]."""
+    """npbx_pack_cols: cols[a][q][j][s] = (A, B)[a][q][j][(0, n-1)[s]], then
+    one word that is 1 exactly when A's and B's boundaries are equal (each
+    face checked with one point differing)."""
     import ctypes
     from npbench_amd import _native, device
     n = 20
@@ -235,12 +284,30 @@ def test_pack_cols_layout(st):
     b = device.to_device(rng.random((n, n, n)), torch.float32)
     elems = ctypes.c_int64()
     _native.call("npbx_cols_elems", n, ctypes.byref(elems))
-    assert elems.value == 4 * n * n
+    assert elems.value == 4 * n * n + 1
     cols = torch.empty(elems.value, dtype=torch.float32, device="cuda")
-    _native.call("npbx_pack_cols", device.dtype_code(a), a.data_ptr(), b.data_ptr(), n,
-                 cols.data_ptr(), None)
-    torch.cuda.synchronize()
+
+    def pack(x, y):
+        _native.call("npbx_pack_cols", device.dtype_code(x), x.data_ptr(), y.data_ptr(), n,
+                     cols.data_ptr(), None)
+        torch.cuda.synchronize()
+        return int(cols[4 * n * n:].view(torch.int32).item())
+
+    assert pack(a, b) == 0
     want = torch.stack([torch.stack([x[:, :, 0], x[:, :, n - 1]], dim=-1) for x in (a, b)])
-    assert torch.equal(cols.view(2, n, n, 2), want)
+    assert torch.equal(cols[:4 * n * n].view(2, n, n, 2), want)
+    A = a.cpu().numpy()
+    B = share_boundary(A, b.cpu().numpy())
+    assert pack(a, device.to_device(B, torch.float32)) == 1
+    assert pack(a, a) == 1
+    for face in FACES:
+        for seed in range(3):
+            C = B.copy()
+            C[face_point(n, face, seed)] += 1.0
+            assert pack(a, device.to_device(C, torch.float32)) == 0, (face, seed)
+        for edge in ((0, 0, 5), (n - 1, 5, n - 1), (4, n - 1, 0)):  # points on edges
+            C = B.copy()
+            C[edge] -= 1.0
+            assert pack(a, device.to_device(C, torch.float32)) == 0, edge
     with pytest.raises(_native.NpbxError):
         _native.call("npbx_pack_cols", 0, a.data_ptr(), b.data_ptr(), n, cols.data_ptr(), None)
