@@ -195,12 +195,15 @@ int npbx_pack_cols(int dtype, const void *A, const void *B, int64_t n,
  * Process-wide knobs read when tile launches are planned (the autotuner in
  * npbench_amd/tune.py sets them): "heat.rows" = rows per wave of a heat tile
  * (2, 4, 6, 8; built for T=2: 2..8 (FP32), 2..6 (FP64), 2 and 4 (BF16); T=3:
- * 4 and 6; T=4: 4; and for FP32 with T=2, 10 and 12, which select the
- * wave-private heat tile: rows per wave including its own 2T halo rows),
+ * 4 and 6; T=4: 4; and for FP32 10 and 12 at T=2, 12 and 14 at T=3, which
+ * select the wave-private heat tile: rows per wave including its own 2T halo
+ * rows),
  * "heat.seg" = default planes per heat tile, "sweep.shape" = the plain
  * sweep's planes per workgroup / planes loaded ahead (0: 16/4, 1: 16/8,
  * 2: 32/8, 3: 8/4), "hdiff.tblock" = iterated hdiff steps fused per HBM round
- * trip (1, the default, 2 or 4; BF16 at most 2).  Results do not depend on them. */
+ * trip (1, the default, 2 or 4; BF16 at most 2), "adi.path" = ADI launches (0,
+ * the default: by size; 1: all time steps in one single-workgroup launch, up
+ * to 512 lines; 2: one launch per half step).  Results do not depend on them. */
 int npbx_set_tuning(const char *key, int64_t value);
 int npbx_get_tuning(const char *key, int64_t *value);
 
@@ -219,6 +222,22 @@ int npbx_hdiff(int dtype, const void *in_field, void *out_field,
 int npbx_vadv(int field_dtype, void *utens_stage, const void *u_stage,
               const void *wcon, const void *u_pos, const void *utens,
               double dtr_stage, int64_t I, int64_t J, int64_t K, void *stream);
+
+/* ---- alternating-direction implicit solver (adi_numpy.py:6-52) -------------
+ * u: (N, N) FP64 (dtype NPBX_F64), updated in place by TSTEPS time steps of
+ * the reference, each a column sweep into library scratch and a row sweep
+ * back into u; bitwise the NumPy result.  N < 3 or TSTEPS < 0 leave u
+ * unchanged; TSTEPS == 0 is an error (the reference divides by it). */
+int npbx_adi(int dtype, void *u, int64_t N, int64_t tsteps, void *stream);
+
+/* Self-check of npbx_adi's short division (no counterpart in the reference;
+ * for tests): `pairs` seeded random FP64 quotients, half of them next to a
+ * rounding midpoint, through the shortened IEEE division sequence the ADI
+ * forward passes use; *checked counts the pairs inside its no-rescale range,
+ * *wrong those whose bits differ from the division operator's.  Synchronous,
+ * on the null stream. */
+int npbx_adi_division_check(int64_t pairs, uint64_t seed, int64_t *checked,
+                            int64_t *wrong);
 
 #ifdef __cplusplus
 }

@@ -124,6 +124,10 @@ int sweep_shape();
 // Fused hdiff steps per round ("hdiff.tblock"; 1 = one launch per step).
 int hdiff_tblock();
 void set_hdiff_tblock(int tb);
+// ADI launch path ("adi.path"; 0 = by size, 1 = every time step in one
+// single-workgroup launch, 2 = one launch per half step).
+int adi_path();
+void set_adi_path(int path);
 
 // Per-device scratch owned by the library (grow-only, freed at exit).  Used
 // by kernels that need storage beyond their inputs and outputs.

@@ -1420,8 +1420,15 @@ extern "C" int npbx_set_tuning(const char* key, int64_t value) {
     set_hdiff_tblock(static_cast<int>(value));
     return NPBX_OK;
   }
+  if (std::strcmp(key, "adi.path") == 0) {
+    NPBX_REQUIRE(value >= 0 && value <= 2, "set_tuning: adi.path must be 0, 1 or 2");
+    set_adi_path(static_cast<int>(value));
+    return NPBX_OK;
+  }
   return fail(NPBX_EINVAL,
-              "set_tuning: unknown key '%s' (heat.rows, heat.seg, sweep.shape, hdiff.tblock)", key);
+              "set_tuning: unknown key '%s' (heat.rows, heat.seg, sweep.shape, hdiff.tblock, "
+              "adi.path)",
+              key);
 }
 
 extern "C" int npbx_get_tuning(const char* key, int64_t* value) {
@@ -1436,6 +1443,10 @@ extern "C" int npbx_get_tuning(const char* key, int64_t* value) {
   }
   if (std::strcmp(key, "sweep.shape") == 0) {
     *value = g_sweep_shape.load();
+    return NPBX_OK;
+  }
+  if (std::strcmp(key, "adi.path") == 0) {
+    *value = adi_path();
     return NPBX_OK;
   }
   if (std::strcmp(key, "hdiff.tblock") == 0) {

@@ -3,10 +3,10 @@
 Two layers:
 
 * resident operations (``jacobi2d_sweeps``, ``heat3d_sweeps``, ``hdiff``,
-  ``vadv``) take device tensors already in their storage dtype (float64,
+  ``vadv``, ``adi``) take device tensors already in their storage dtype (float64,
   float32 or bfloat16) and enqueue HIP kernels on a stream without waiting;
 * drop-in operations (``jacobi_2d``, ``heat_3d``, ``hdiff_dropin``,
-  ``vadv_dropin``) take the reference's float64 arrays -- device tensors
+  ``vadv_dropin``, ``adi_dropin``) take the reference's float64 arrays -- device tensors
   produced by the framework's copy function, or host NumPy arrays, which are
   copied to the device, processed and copied back in place -- and a
   ``precision`` that picks the storage form.  FP32/BF16 forms convert to the
@@ -72,6 +72,16 @@ def vadv(utens_stage, u_stage, wcon, u_pos, utens, dtr_stage, stream=None):
     dev.check_device_array("wcon", wcon, shape=(I + 1, J, K), dtype=dt)
     nat.call("npbx_vadv", dev.dtype_code(utens_stage), dev.ptr(utens_stage), dev.ptr(u_stage),
              dev.ptr(wcon), dev.ptr(u_pos), dev.ptr(utens), float(dtr_stage), I, J, K,
+             dev.stream_handle(stream))
+
+
+def adi(u, tsteps, stream=None):
+    """``tsteps`` ADI time steps (adi_numpy.py:6-52) of a float64 (N, N)
+    device tensor, in place."""
+    dev.check_device_array("u", u, dtype=torch.float64)
+    if u.dim() != 2 or u.shape[0] != u.shape[1]:
+        raise ValueError("adi: u must be a square (N, N) array")
+    nat.call("npbx_adi", dev.dtype_code(u), dev.ptr(u), u.shape[0], int(tsteps),
              dev.stream_handle(stream))
 
 
@@ -164,3 +174,12 @@ def vadv_dropin(utens_stage, u_stage, wcon, u_pos, utens, dtr_stage, precision="
         raise ValueError("vadv supports fp64 and fp32 field storage")
     with _Staged(precision, [utens_stage, u_stage, wcon, u_pos, utens], outputs=(0,)) as w:
         vadv(w[0], w[1], w[2], w[3], w[4], dtr_stage)
+
+
+def adi_dropin(TSTEPS, N, u):
+    """adi_numpy.kernel(TSTEPS, N, u): u updated in place and returned."""
+    if tuple(u.shape) != (N, N):
+        raise ValueError(f"adi: u has shape {tuple(u.shape)}, expected ({N}, {N})")
+    with _Staged("fp64", [u], outputs=(0,)) as (w,):
+        adi(w, TSTEPS)
+    return u

@@ -35,4 +35,5 @@ from .stencils import (  # noqa: F401
     vadv,
     jacobi_sweeps,
     heat_sweeps,
+    adi_kernel,
 )

@@ -8,7 +8,8 @@ operand in ``oracle.stencils`` shows up as a bit difference.  Each function
 returns new arrays and leaves its inputs alone.
 
 References: jacobi_2d_numpy.py:4-10, heat_3d_numpy.py:4-20,
-hdiff_numpy.py:5-29, vadv_numpy.py:4-78 (npbench/benchmarks/...).
+hdiff_numpy.py:5-29, vadv_numpy.py:4-78, adi_numpy.py:6-52
+(npbench/benchmarks/...).
 """
 
 import numpy as np
@@ -173,3 +174,43 @@ def vadv(utens_stage, u_stage, wcon, u_pos, utens, dtr_stage):
             for k in range(K):
                 out[i, j, k] = dtr_stage * (x[k] - u_pos[i, j, k])
     return out
+
+
+def adi(TSTEPS, N, u):
+    """adi_numpy.kernel point by point: each line's p, q and back substitution
+    with its own scalar recurrence (p recomputed per line, as the reference
+    does), Python floats throughout.  Returns the new u."""
+    n = N
+    u = [[float(x) for x in row] for row in np.asarray(u)]
+    dx = dy = 1.0 / n
+    dt = 1.0 / TSTEPS
+    mul1 = 2.0 * dt / (dx * dx)
+    mul2 = 1.0 * dt / (dy * dy)
+    a = c = -mul1 / 2.0
+    b = e = 1.0 + mul2
+    d = f = -mul2 / 2.0
+    v = [[0.0] * n for _ in range(n)]
+    for _ in range(TSTEPS):
+        for i in range(1, n - 1):  # column i of u -> column i of v
+            v[0][i] = 1.0
+            p, q = [0.0], [v[0][i]]
+            for j in range(1, n - 1):
+                den = a * p[j - 1] + b
+                p.append(-c / den)
+                rhs = -d * u[j][i - 1] + (1.0 + 2.0 * d) * u[j][i] - f * u[j][i + 1]
+                q.append((rhs - a * q[j - 1]) / den)
+            v[n - 1][i] = 1.0
+            for j in range(n - 2, 0, -1):
+                v[j][i] = p[j] * v[j + 1][i] + q[j]
+        for i in range(1, n - 1):  # row i of v -> row i of u
+            u[i][0] = 1.0
+            p, q = [0.0], [u[i][0]]
+            for j in range(1, n - 1):
+                den = d * p[j - 1] + e
+                p.append(-f / den)
+                rhs = -a * v[i - 1][j] + (1.0 + 2.0 * a) * v[i][j] - c * v[i + 1][j]
+                q.append((rhs - d * q[j - 1]) / den)
+            u[i][n - 1] = 1.0
+            for j in range(n - 2, 0, -1):
+                u[i][j] = p[j] * u[i][j + 1] + q[j]
+    return np.array(u)

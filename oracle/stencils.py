@@ -9,6 +9,7 @@ Reference files (paths relative to the npbench source tree):
   heat_3d_kernel    npbench/benchmarks/polybench/heat_3d/heat_3d_numpy.py:4-20
   hdiff             npbench/benchmarks/weather_stencils/hdiff/hdiff_numpy.py:5-29
   vadv              npbench/benchmarks/weather_stencils/vadv/vadv_numpy.py:4-78
+  adi_kernel        npbench/benchmarks/polybench/adi/adi_numpy.py:6-52
 """
 
 import numpy as np
@@ -233,3 +234,81 @@ def vadv(utens_stage, u_stage, wcon, u_pos, utens, dtr_stage):
     for k in range(K - 2, -1, -1):
         x = dcol[:, :, k] - ccol[:, :, k] * x
         utens_stage[:, :, k] = dtr_stage * (x - u_pos[:, :, k])
+
+
+# --------------------------------------------------------------------------
+# Alternating-direction implicit solver (polybench ADI)
+# --------------------------------------------------------------------------
+def adi_coefficients(TSTEPS, N):
+    """adi_numpy.py:11-25: (a, b, c, d, e, f) as the reference forms them.
+
+    Note b = 1 + mul2 (not mul1), as the reference has it."""
+    dx = 1.0 / N
+    dy = 1.0 / N
+    dt = 1.0 / TSTEPS
+    mul1 = 2.0 * dt / (dx * dx)
+    mul2 = 1.0 * dt / (dy * dy)
+    a = -mul1 / 2.0
+    d = -mul2 / 2.0
+    return a, 1.0 + mul2, a, d, 1.0 + mul2, d
+
+
+def _adi_p(lo, mid, hi, steps):
+    """p[0] = 0, p[j] = -hi / (lo * p[j-1] + mid) and the denominators, as
+    Python floats: the reference computes the same values once per line
+    (adi_numpy.py:28,30 and :38,40); every line's p is the same."""
+    p, den = [0.0], [None]
+    for _ in range(steps):
+        den.append(lo * p[-1] + mid)
+        p.append(-hi / den[-1])
+    return p, den
+
+
+def adi_column_sweep(u, v, coef):
+    """adi_numpy.py:27-34 with coef = (a, b, c, d, f): for every interior
+    column i, solve along the rows into v[:, i] (v's rows 0 and N-1 set to 1);
+    right-hand side ((-d u[j,i-1] + (1+2d) u[j,i]) - f u[j,i+1])."""
+    a, b, c, d, f = coef
+    n = u.shape[0]
+    p, den = _adi_p(a, b, c, n - 2)
+    v[0, 1:n - 1] = 1.0
+    v[n - 1, 1:n - 1] = 1.0
+    q = np.empty((n - 1, n - 2))
+    q[0] = v[0, 1:n - 1]
+    for j in range(1, n - 1):
+        rhs = -d * u[j, 0:n - 2] + (1.0 + 2.0 * d) * u[j, 1:n - 1] - f * u[j, 2:n]
+        q[j] = (rhs - a * q[j - 1]) / den[j]
+    for j in range(n - 2, 0, -1):
+        v[j, 1:n - 1] = p[j] * v[j + 1, 1:n - 1] + q[j]
+
+
+def adi_row_sweep(v, u, coef):
+    """adi_numpy.py:36-46 with coef = (a, c, d, e, f): for every interior row
+    i, solve along the columns into u[i, :] (u's columns 0 and N-1 set to 1);
+    right-hand side ((-a v[i-1,j] + (1+2a) v[i,j]) - c v[i+1,j])."""
+    a, c, d, e, f = coef
+    n = u.shape[0]
+    p, den = _adi_p(d, e, f, n - 2)
+    u[1:n - 1, 0] = 1.0
+    u[1:n - 1, n - 1] = 1.0
+    q = np.empty((n - 1, n - 2))
+    q[0] = u[1:n - 1, 0]
+    for j in range(1, n - 1):
+        rhs = -a * v[0:n - 2, j] + (1.0 + 2.0 * a) * v[1:n - 1, j] - c * v[2:n, j]
+        q[j] = (rhs - d * q[j - 1]) / den[j]
+    for j in range(n - 2, 0, -1):
+        u[1:n - 1, j] = p[j] * u[1:n - 1, j + 1] + q[j]
+
+
+def adi_kernel(TSTEPS, N, u):
+    """adi_numpy.kernel(TSTEPS, N, u): TSTEPS column and row sweeps, u updated
+    in place and returned."""
+    _f64(u)
+    if u.shape != (N, N):
+        raise ValueError("adi: u must be (N, N)")
+    a, b, c, d, e, f = adi_coefficients(TSTEPS, N)
+    v = np.empty_like(u)
+    for _ in range(1, TSTEPS + 1):
+        adi_column_sweep(u, v, (a, b, c, d, f))
+        adi_row_sweep(v, u, (a, c, d, e, f))
+    return u

@@ -17,6 +17,8 @@ Cases (all FP64):
   hdiff_it    (4,4,3) with nsteps = 3 (build-defined iteration)
   vadv        (2,3,6) reference init (wcon in [0,1)) and config init
               (wcon in [-1,1)), and the K=2 edge case
+  adi         N=9  TSTEPS=3 reference init (fromfunction); N=6 TSTEPS=2
+              seeded uniform
 """
 
 import os
@@ -33,6 +35,7 @@ from npbench_amd.benchmarks.polybench.jacobi_2d import jacobi_2d  # noqa: E402
 from npbench_amd.benchmarks.polybench.heat_3d import heat_3d  # noqa: E402
 from npbench_amd.benchmarks.weather_stencils.hdiff import hdiff  # noqa: E402
 from npbench_amd.benchmarks.weather_stencils.vadv import vadv  # noqa: E402
+from npbench_amd.benchmarks.polybench.adi import adi  # noqa: E402
 
 
 def save(name, **arrays):
@@ -77,6 +80,12 @@ def main():
         out = scalar.vadv(us, u, w, up, ut, dtr)
         save(f"vadv_{tag}", dtr_stage=dtr, utens_stage=us, u_stage=u, wcon=w, u_pos=up,
              utens=ut, out=out)
+
+    # adi
+    for tag, n, tsteps, u in (("ref", 9, 3, adi.initialize(9)),
+                              ("uniform", 6, 2, np.random.default_rng(5).random((6, 6)))):
+        out = scalar.adi(tsteps, n, u)
+        save(f"adi_{tag}", TSTEPS=tsteps, N=n, u=u, out=out)
 
 
 if __name__ == "__main__":

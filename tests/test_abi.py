@@ -54,6 +54,10 @@ def test_version_string():
     (lambda: nat.call("npbx_vadv", 0, None, None, None, None, None, 0.15, 2, 2, 1, None), "K >= 2"),
     (lambda: nat.call("npbx_cast", 0, None, 7, None, 10, None), "null"),
     (lambda: nat.call("npbx_jacobi2d", 9, 1, 2, 10, 3, 1, None), "dtype"),
+    (lambda: nat.call("npbx_adi", 0, None, 10, 0, None), "TSTEPS=0"),
+    (lambda: nat.call("npbx_adi", 0, None, 0, 5, None), "N=0"),
+    (lambda: nat.call("npbx_adi", 1, None, 10, 5, None), "dtype"),
+    (lambda: nat.call("npbx_adi", 0, None, 10, 5, None), "null"),
 ])
 def test_argument_errors_raise_with_message(call, needle):
     with pytest.raises(nat.NpbxError) as ei:
@@ -67,6 +71,8 @@ def test_empty_work_is_a_no_op():
     assert nat.call("npbx_jacobi2d", 0, None, None, 2, 10, 1, None) == nat.OK
     assert nat.call("npbx_heat3d", 0, None, None, 10, 1, 1, None) == nat.OK
     assert nat.call("npbx_cast", 0, None, 1, None, 0, None) == nat.OK
+    assert nat.call("npbx_adi", 0, None, 2, 5, None) == nat.OK    # no interior line
+    assert nat.call("npbx_adi", 0, None, 10, -3, None) == nat.OK   # no time step
 
 
 def test_last_error_buffer_is_terminated():

@@ -21,6 +21,12 @@ fails at least one check:
              gives utens + utens_stage; the elimination matches a banded LU
              solve (scipy.linalg.solve_banded); linearity; brute force and
              golden vectors (including K = 2).
+  adi        each half step solves the tridiagonal systems it stands for
+             (scipy.linalg.solve_banded, with coefficients that tell a, b, c,
+             d, e, f apart); whole time steps against solves composed by the
+             test; the reference's coefficients (b = 1 + mul2); untouched
+             rows 0 and N-1 and the boundary columns set to 1; brute force
+             and golden vectors.
 
 The last section plants mistakes in the oracle (monkeypatched copies) and
 checks the pins above catch them.
@@ -333,6 +339,105 @@ def test_vadv_rejects_k1():
         S.vadv(z.copy(), z, np.zeros((2, 1, 1)), z, z, 0.15)
 
 
+# ------------------------------------------------------------------ adi
+# distinct weights, diagonally dominant: a swapped or wrong weight shows
+ADI_GENERAL = dict(a=-0.3, b=1.9, c=-0.45, d=-0.2, e=1.7, f=-0.35)
+
+
+def _tridiag_solve(lo, mid, hi, rhs, first, last):
+    """x[1:-1] of lo x[j-1] + mid x[j] + hi x[j+1] = rhs[j] (j = 1 .. m),
+    x[0] = first, x[m+1] = last, by a banded LU solve."""
+    import scipy.linalg as la
+    m = len(rhs)
+    ab = np.zeros((3, m))
+    ab[0, 1:] = hi
+    ab[1, :] = mid
+    ab[2, :-1] = lo
+    r = np.array(rhs, dtype=np.float64)
+    r[0] -= lo * first
+    r[-1] -= hi * last
+    return la.solve_banded((1, 1), ab, r)
+
+
+def adi_columns_expected(u, a, b, c, d, f):
+    n = u.shape[0]
+    v = np.full((n, n), np.nan)
+    v[0, 1:-1] = v[-1, 1:-1] = 1.0
+    for i in range(1, n - 1):
+        rhs = -d * u[1:-1, i - 1] + (1 + 2 * d) * u[1:-1, i] - f * u[1:-1, i + 1]
+        v[1:-1, i] = _tridiag_solve(a, b, c, rhs, 1.0, 1.0)
+    return v
+
+
+def adi_rows_expected(v, u0, a, c, d, e, f):
+    n = v.shape[0]
+    u = u0.copy()
+    u[1:-1, 0] = u[1:-1, -1] = 1.0
+    for i in range(1, n - 1):
+        rhs = -a * v[i - 1, 1:-1] + (1 + 2 * a) * v[i, 1:-1] - c * v[i + 1, 1:-1]
+        u[i, 1:-1] = _tridiag_solve(d, e, f, rhs, 1.0, 1.0)
+    return u
+
+
+@pytest.mark.parametrize("n", [3, 4, 9, 24])
+def test_adi_column_sweep_solves_its_systems(n):
+    k = ADI_GENERAL
+    u = np.random.default_rng(n).random((n, n))
+    v = np.full((n, n), np.nan)
+    S.adi_column_sweep(u, v, (k["a"], k["b"], k["c"], k["d"], k["f"]))
+    want = adi_columns_expected(u, k["a"], k["b"], k["c"], k["d"], k["f"])
+    assert np.allclose(v[:, 1:-1], want[:, 1:-1], rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("n", [3, 4, 9, 24])
+def test_adi_row_sweep_solves_its_systems(n):
+    k = ADI_GENERAL
+    rng = np.random.default_rng(100 + n)
+    v, u0 = rng.random((n, n)), rng.random((n, n))
+    u = u0.copy()
+    S.adi_row_sweep(v, u, (k["a"], k["c"], k["d"], k["e"], k["f"]))
+    want = adi_rows_expected(v, u0, k["a"], k["c"], k["d"], k["e"], k["f"])
+    assert np.allclose(u, want, rtol=1e-13, atol=1e-14)
+    assert np.array_equal(u[0], u0[0]) and np.array_equal(u[-1], u0[-1])
+
+
+def test_adi_coefficients():
+    a, b, c, d, e, f = S.adi_coefficients(4, 10)
+    # DT = 1/4, DX = DY = 1/10: mul1 = 2*DT/DX^2 = 50, mul2 = DT/DY^2 = 25
+    assert a == pytest.approx(-25.0, rel=1e-14) and c == a
+    assert d == pytest.approx(-12.5, rel=1e-14) and f == d
+    assert b == pytest.approx(26.0, rel=1e-14) and e == b  # b = 1 + mul2
+
+
+@pytest.mark.parametrize("n,tsteps", [(5, 1), (8, 2), (12, 3)])
+def test_adi_kernel_is_composed_solves(n, tsteps):
+    """Whole time steps against the test's own solves: the first column
+    sweep reads u's original boundary columns, later ones read 1."""
+    a, b, c, d, e, f = S.adi_coefficients(tsteps, n)
+    u0 = np.random.default_rng(n).random((n, n))
+    want = u0.copy()
+    for _ in range(tsteps):
+        v = adi_columns_expected(want, a, b, c, d, f)
+        want = adi_rows_expected(v, want, a, c, d, e, f)
+    got = S.adi_kernel(tsteps, n, u0.copy())
+    scale = np.max(np.abs(want))
+    assert np.max(np.abs(got - want)) <= 1e-11 * scale
+
+
+def test_adi_boundaries():
+    n = 10
+    u0 = np.random.default_rng(3).random((n, n))
+    u = S.adi_kernel(2, n, u0.copy())
+    assert np.array_equal(u[0], u0[0]) and np.array_equal(u[-1], u0[-1])
+    assert np.all(u[1:-1, 0] == 1.0) and np.all(u[1:-1, -1] == 1.0)
+
+
+@pytest.mark.parametrize("n,tsteps", [(3, 1), (4, 3), (7, 2), (11, 4)])
+def test_adi_bruteforce_bitwise(n, tsteps):
+    u = np.random.default_rng(7 * n + tsteps).random((n, n))
+    assert np.array_equal(S.adi_kernel(tsteps, n, u.copy()), scalar.adi(tsteps, n, u))
+
+
 # ------------------------------------------------------------------ golden
 def _golden(name):
     with np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False) as z:
@@ -366,6 +471,14 @@ def test_golden_hdiff(name):
     out = g["out_field"].copy()
     S.hdiff_iterated(g["in_field"], out, g["coeff"], int(g.get("nsteps", 1)))
     assert np.array_equal(out, g["out"])
+
+
+@pytest.mark.parametrize("name", ["adi_ref", "adi_uniform"])
+def test_golden_adi(name):
+    g = _golden(name)
+    u = g["u"].copy()
+    S.adi_kernel(int(g["TSTEPS"]), int(g["N"]), u)
+    assert np.array_equal(u, g["out"])
 
 
 @pytest.mark.parametrize("name", ["vadv_ref", "vadv_config", "vadv_k2"])
@@ -478,3 +591,28 @@ def test_planted_vadv_mistakes_are_caught(monkeypatch):
     # an explicit correction with the wrong sign breaks it as well
     monkeypatch.setattr(S, "BET_M", -0.5)
     assert vadv_exact_err() > 1e-6
+
+
+@pytest.mark.parametrize("mistake", ["swap_neighbours", "rhs_weight", "p_order"])
+def test_planted_adi_mistakes_are_caught(monkeypatch, mistake):
+    """Mutant column sweeps (neighbours u[j,i-1] and u[j,i+1] swapped; the
+    centre weight 1+2d written 1+d; p's recurrence with b and a swapped) fail
+    the banded-solve check."""
+    def mutant(u, v, coef):
+        a, b, c, d, f = coef
+        n = u.shape[0]
+        lo, mid = (b, a) if mistake == "p_order" else (a, b)
+        p, den = S._adi_p(lo, mid, c, n - 2)
+        v[0, 1:n - 1] = v[n - 1, 1:n - 1] = 1.0
+        q = np.empty((n - 1, n - 2))
+        q[0] = 1.0
+        w = 1.0 + (d if mistake == "rhs_weight" else 2.0 * d)
+        west, east = (u[:, 2:], u[:, :-2]) if mistake == "swap_neighbours" else (u[:, :-2], u[:, 2:])
+        for j in range(1, n - 1):
+            rhs = -d * west[j] + w * u[j, 1:n - 1] - f * east[j]
+            q[j] = (rhs - a * q[j - 1]) / den[j]
+        for j in range(n - 2, 0, -1):
+            v[j, 1:n - 1] = p[j] * v[j + 1, 1:n - 1] + q[j]
+    monkeypatch.setattr(S, "adi_column_sweep", mutant)
+    with pytest.raises(AssertionError):
+        test_adi_column_sweep_solves_its_systems(9)
