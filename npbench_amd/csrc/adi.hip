@@ -70,14 +70,16 @@ AdiCoef adi_coef(int64_t tsteps, int64_t n) {
   return k;
 }
 
-// Tables of one sweep, n entries each (entry j for step j = 1 .. n-2).
+// Tables of one sweep, tl >= n entries each (entry j for step j = 1 .. n-2;
+// entries past n-2 repeat entry n-2, so a 16-step chunk reads a contiguous
+// block without clamping).
 struct SweepTab {
   const double* p;    // p[j]
   const double* den;  // the forward denominators, lo p[j-1] + mid
   const double* y;    // the division sequence's reciprocal of den[j]
   const double* ok;   // ok[0] = 1 when every den[j] is in den_in_range;
-                      // ok[1], ok[2]: numerators n with ok[1] <= |n| < ok[2]
-                      // never make the sequence rescale, for any den[j]
+                      // ok[1], ok[2]: quotient estimates m = n y with
+                      // ok[1] <= |m| < ok[2] never make it rescale
 };
 
 struct AdiArgs {
@@ -110,8 +112,8 @@ __device__ __forceinline__ double division_reciprocal(double den) {
   return __builtin_fma(y, e, y);
 }
 
-__device__ __forceinline__ double short_division(double n, double den, double y) {
-  const double m = n * y;
+__device__ __forceinline__ double short_division(double n, double den, double y, double& m) {
+  m = n * y;
   const double r = __builtin_fma(-den, m, n);
   return __builtin_fma(r, y, m);
 }
@@ -127,38 +129,40 @@ __device__ __forceinline__ bool den_in_range(double den) {
   return e >= 100 && e <= 1900;
 }
 
-// The numerator's side, for dens in range between dmin and dmax in
-// magnitude: the division sequence rescales when n is zero, tiny (biased
-// exponent <= 53), not finite, at least 2^768 times den (in exponent), or
-// when the quotient is subnormal.  lo <= |n| < hi excludes all of them:
-//   lo = max(2^-960, 2^-1020 dmax), hi = 2^766 dmin
-// (a NaN fails both comparisons; hi may be infinite when dmin is large,
-// where no finite n reaches the exponent gap).
-__device__ __forceinline__ void numerator_bounds(double dmin, double dmax, double& lo,
-                                                 double& hi) {
-  lo = fmax(0x1p-960, 0x1p-1020 * dmax);
-  hi = 0x1p766 * dmin;
+// The numerator's side, for dens in range of magnitude dmin or more: the
+// division sequence rescales when n is zero, tiny (biased exponent <= 53),
+// not finite, at least 2^768 times den (in exponent), or when the quotient
+// is subnormal.  With m = n y, the quotient to within a few units of its
+// last place, lo <= |m| < hi excludes all of them:
+//   hi = 2^765: |n / den| < 2^766, so the exponents differ by at most 766;
+//   lo = max(2^-1020, 2^-960 / dmin): the quotient is normal, and
+//        |n| = |m den| >= 2^-960 (biased exponent above 53).
+// A NaN fails both comparisons, an infinite n gives an infinite m.
+__device__ __forceinline__ void quotient_bounds(double dmin, double& lo, double& hi) {
+  lo = fmax(0x1p-1020, 0x1p-960 / dmin);
+  hi = 0x1p765;
 }
 
-__device__ __forceinline__ bool unscaled_division(double n, double lo, double hi) {
-  const double a = fabs(n);
+__device__ __forceinline__ bool unscaled_division(double m, double lo, double hi) {
+  const double a = fabs(m);
   return a >= lo && a < hi;
 }
 
 // p[j] = -c / (a p[j-1] + b) with p[0] = 0 (adi_numpy.py:28,30) and
 // p[j] = -f / (d p[j-1] + e) (:38,40), for j = 1 .. n-2 (lane 0: the column
-// sweep's table, lane 1: the row sweep's), with each den's reciprocal.
-__global__ void adi_tables(AdiCoef k, int n, double* __restrict__ tab) {
+// sweep's table, lane 1: the row sweep's), with each den's reciprocal; each
+// table tl entries long.
+__global__ void adi_tables(AdiCoef k, int n, int tl, double* __restrict__ tab) {
   const int lane = threadIdx.x;
   if (lane > 1) return;
   const double lo = lane == 0 ? k.a : k.d;
   const double mid = lane == 0 ? k.b : k.e;
   const double hi = lane == 0 ? k.c : k.f;
-  double* p = tab + (lane * 4) * n;
-  double* den = p + n;
-  double* y = den + n;
-  double* ok = y + n;
-  double prev = 0.0, dmin = INFINITY, dmax = 0.0;
+  double* p = tab + (lane * 4) * tl;
+  double* den = p + tl;
+  double* y = den + tl;
+  double* ok = y + tl;
+  double prev = 0.0, dmin = INFINITY;
   bool in_range = true;
   p[0] = den[0] = y[0] = 0.0;
   for (int j = 1; j < n - 1; ++j) {
@@ -169,10 +173,14 @@ __global__ void adi_tables(AdiCoef k, int n, double* __restrict__ tab) {
     y[j] = division_reciprocal(dj);
     in_range = in_range && den_in_range(dj);
     dmin = fmin(dmin, fabs(dj));
-    dmax = fmax(dmax, fabs(dj));
+  }
+  for (int j = n - 1; j < tl; ++j) {
+    p[j] = p[n - 2];
+    den[j] = den[n - 2];
+    y[j] = y[n - 2];
   }
   ok[0] = in_range ? 1.0 : 0.0;
-  numerator_bounds(dmin, dmax, ok[1], ok[2]);
+  quotient_bounds(dmin, ok[1], ok[2]);
 }
 
 // LDS of one workgroup: two buffers, each a tile of at most (NT + 2) x
@@ -191,10 +199,11 @@ __device__ __forceinline__ ConstTab as_const(const double* p) {
   return (ConstTab)(p);  // address-space cast: C-style casts only
 }
 
-// Entries j0 .. j0+kCW-1 of a table (indices past n-2 read entry n-2).
-__device__ __forceinline__ void table_chunk(ConstTab tab, int n, int j0, double (&d)[kCW]) {
+// Entries j0 .. j0+kCW-1 of a table: one contiguous block.
+__device__ __forceinline__ void table_chunk(ConstTab tab, int j0, double (&d)[kCW]) {
+  const ConstTab b = tab + j0;
 #pragma unroll
-  for (int kk = 0; kk < kCW; ++kk) d[kk] = tab[min(j0 + kk, n - 2)];
+  for (int kk = 0; kk < kCW; ++kk) d[kk] = b[kk];
 }
 
 // Row j of a [row][line] scratch array: a wave-uniform base, so that the
@@ -203,12 +212,35 @@ __device__ __forceinline__ double* row_of(double* base, int j, int pitch) {
   return base + static_cast<uint32_t>(j * pitch);
 }
 
+// Where a line keeps q between its forward pass and its back substitution:
+// step j at base[j * pitch] -- the [step][line] scratch in device memory, or,
+// for one-wave workgroups on grids of at most kQRows steps, an LDS array of
+// the same layout (no device-memory round trip per step).
+struct QSlots {
+  double* base;
+  int pitch;
+  __device__ __forceinline__ double& at(int j) const {
+    return base[static_cast<uint32_t>(j * pitch)];
+  }
+};
+constexpr int kQRows = 280;
+
 // Hides the thread index, the side and the line base from loop-invariant code
 // motion: the one-launch kernel runs both sweeps in a time-step loop, and
 // addresses hoisted out of it would hold both sweeps' registers at once.
 __device__ __forceinline__ void opaque(int& t, int& n, int& lb) {
   asm volatile("" : "+v"(t));
   asm volatile("" : "+s"(n), "+s"(lb));
+}
+
+// A workgroup barrier that orders LDS only.  The chunk loops' barriers guard
+// the LDS tiles; with __syncthreads they would also wait for every global
+// store in flight (the per-step stores of q and v), which the other threads
+// never read within the sweep.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 // Per-sweep constants of the forward pass.
@@ -231,43 +263,60 @@ __device__ __forceinline__ Fwd forward_consts(const SweepTab& tab, double w) {
 // steps j = j0 + jj, stored to row j of qs.  rhs(jj) reads the chunk's LDS
 // tile, dy the chunk's (den, y) pairs.  Steps past n-2 compute values that
 // nothing reads.
+// The chunk's right-hand sides and table pairs are read from LDS and formed
+// before the recurrence starts: none depends on q, and with them in registers
+// the chain waits for nothing but itself.
 template <class Rhs>
-__device__ __forceinline__ void forward_chunk(double& q, const Fwd& f, const double* dy, int n,
-                                              int j0, bool live, double* qs, int ld, int col,
-                                              Rhs rhs) {
+__device__ __forceinline__ void forward_chunk(double& q, const Fwd& f, const double* dy, int j0,
+                                              const QSlots& qs, Rhs rhs) {
+  double rv[kCW], den[kCW], y[kCW];
+#pragma unroll
+  for (int jj = 0; jj < kCW; ++jj) {
+    const double2 d = *reinterpret_cast<const double2*>(dy + 2 * jj);
+    den[jj] = d.x;
+    y[jj] = d.y;
+    rv[jj] = rhs(jj);
+  }
   const double q0 = q;
   bool redo = !f.fast;
   if (f.fast) {
-    bool bad = false;
+    // the quotient estimates' magnitude range over the chunk; lanes past
+    // the last line and steps past n-2 take part too (their values come from
+    // clamped loads of the grid, so they only rarely cause a redo).  A NaN
+    // (from a NaN in u) is left out and stays NaN in both forms.
+    double amin = INFINITY, amax = 0.0;
 #pragma unroll
     for (int jj = 0; jj < kCW; ++jj) {
-      const double2 d = *reinterpret_cast<const double2*>(dy + 2 * jj);
-      const double num = rhs(jj) - f.w * q;
-      q = short_division(num, d.x, d.y);
-      row_of(qs, j0 + jj, ld)[col] = q;
-      bad = bad || (live && j0 + jj <= n - 2 && !unscaled_division(num, f.lo, f.hi));
+      const double num = rv[jj] - f.w * q;
+      double m;
+      q = short_division(num, den[jj], y[jj], m);
+      qs.at(j0 + jj) = q;
+      amin = fmin(amin, fabs(m));
+      amax = fmax(amax, fabs(m));
     }
-    redo = __any(bad);
+    redo = __any(!(amin >= f.lo && amax < f.hi));
   }
   if (redo) {
     q = q0;
 #pragma unroll
     for (int jj = 0; jj < kCW; ++jj) {
-      q = (rhs(jj) - f.w * q) / dy[2 * jj];
-      row_of(qs, j0 + jj, ld)[col] = q;
+      q = (rv[jj] - f.w * q) / den[jj];
+      qs.at(j0 + jj) = q;
     }
   }
 }
 
-// The table pairs of chunk j0 for the LDS buffer: threads 32..63 each load
-// one entry (den of step j0 + (t-32), then y), clamped to step n-2.
-__device__ __forceinline__ double fetch_pair(const SweepTab& tab, int t, int n, int j0) {
-  const int e = t - 32;
-  const int j = min(j0 + (e & (kCW - 1)), n - 2);
-  return (e < kCW ? tab.den : tab.y)[j];
+// Besides its kCW tile values, each of the lanes 0..63 loads one more value
+// per chunk: lanes 0..31 the tile's last two columns or rows, lanes 32..63
+// the chunk's table pairs (den of step j0 + (t-32), then y).  One load from
+// a per-lane address, so that nothing is chosen by a branch (a branch here
+// made the compiler keep the two values in a stack array).
+__device__ __forceinline__ const double* pair_addr(const SweepTab& tab, int t, int j0) {
+  const int e = (t - 2 * kCW) & (2 * kCW - 1);
+  return (e < kCW ? tab.den : tab.y) + j0 + (e & (kCW - 1));
 }
 __device__ __forceinline__ void commit_pair(double* dy, int t, double val) {
-  const int e = t - 32;
+  const int e = t - 2 * kCW;
   if (e >= 0 && e < 2 * kCW) dy[2 * (e & (kCW - 1)) + (e >= kCW ? 1 : 0)] = val;
 }
 
@@ -289,23 +338,61 @@ __device__ __forceinline__ void back_chunk(double& x, const double (&q)[kCW],
 }
 
 // This thread's q of steps j0 .. j0+kCW-1.
-__device__ __forceinline__ void q_chunk(double* qs, int ld, int col, int j0, double (&q)[kCW]) {
+__device__ __forceinline__ void q_chunk(const QSlots& qs, int j0, double (&q)[kCW]) {
 #pragma unroll
-  for (int kk = 0; kk < kCW; ++kk) q[kk] = row_of(qs, j0 + kk, ld)[col];
+  for (int kk = 0; kk < kCW; ++kk) q[kk] = qs.at(j0 + kk);
+}
+
+// Runs process(c, q, partial) for the chunks c = nchunks-1 down to 0, each
+// with its q loaded three chunks ahead into one of four register sets used
+// in turn (no copies; a chunk's 16 steps are too short to cover an L2 round
+// trip); the first chunk processed is the partial one.
+template <class Process>
+__device__ __forceinline__ void back_chunks(int nchunks, const QSlots& qs, Process process) {
+  double qr[4][kCW];
+  auto load = [&](int c, double (&dst)[kCW]) {
+    if (c >= 0) q_chunk(qs, 1 + c * kCW, dst);
+  };
+  int c = nchunks - 1;
+  load(c, qr[0]);
+  load(c - 1, qr[1]);
+  load(c - 2, qr[2]);
+  load(c - 3, qr[3]);
+  process(c, qr[0], std::true_type{});
+  load(c - 4, qr[0]);
+  // chunk c - k sits in set k % 4
+  for (c -= 1; c >= 0; c -= 4) {
+    process(c, qr[1], std::false_type{});
+    load(c - 4, qr[1]);
+    if (c - 1 < 0) break;
+    process(c - 1, qr[2], std::false_type{});
+    load(c - 5, qr[2]);
+    if (c - 2 < 0) break;
+    process(c - 2, qr[3], std::false_type{});
+    load(c - 6, qr[3]);
+    if (c - 3 < 0) break;
+    process(c - 3, qr[0], std::false_type{});
+    load(c - 7, qr[0]);
+  }
 }
 
 // Column sweep (adi_numpy.py:27-34) of the lines lb+1 .. lb+NT: column i
 // of u solved along the rows into column i of v.
-template <int NT>
-__device__ __forceinline__ void column_sweep(const AdiArgs& g, int lb, double* lds) {
+template <int NT, bool QLDS>
+__device__ __forceinline__ void column_sweep(const AdiArgs& g, int lb, double* lds,
+                                             double* qlds) {
   using L = AdiLds<NT>;
   constexpr int W = NT + 2;  // tile columns lb .. lb+NT+1
   int t = threadIdx.x, n = g.n;
   opaque(t, n, lb);
   const int i = lb + 1 + t;
-  const bool live = i <= n - 2;
   const int nchunks = (n - 2 + kCW - 1) / kCW;
   const Fwd f = forward_consts(g.cols, g.a);
+  QSlots qs;
+  if constexpr (QLDS)
+    qs = {qlds + t, NT};
+  else
+    qs = {g.qs + lb + t, g.ld};
   // Tile of rows j0 .. j0+kCW-1 of u, columns lb .. lb+W-1, tile[row][col]:
   // thread t loads column lb+t of every row, threads 0..31 also the last two
   // columns (row j0 + t/2, column lb+NT + t%2).  Rows and columns are
@@ -313,13 +400,13 @@ __device__ __forceinline__ void column_sweep(const AdiArgs& g, int lb, double* l
   // n-2 and lanes past the last line.
   const int col = min(lb + t, n - 1);
   const int xr = t >> 1, xcol = min(lb + NT + (t & 1), n - 1);
-  double pre[kCW], prex = 0.0, prey = 0.0;
+  double pre[kCW], extra;
   auto fetch = [&](int c) {
     const int j0 = 1 + c * kCW;
 #pragma unroll
     for (int m = 0; m < kCW; ++m) pre[m] = g.u[static_cast<int64_t>(min(j0 + m, n - 2)) * n + col];
-    if (t < 2 * kCW) prex = g.u[static_cast<int64_t>(min(j0 + xr, n - 2)) * n + xcol];
-    else if (t < 4 * kCW) prey = fetch_pair(g.cols, t, n, j0);
+    const double* xa = g.u + static_cast<int64_t>(min(j0 + (xr & (kCW - 1)), n - 2)) * n + xcol;
+    extra = *(t < 2 * kCW ? xa : pair_addr(g.cols, t, j0));
   };
   double q = 1.0;  // q[0] = v[0, i] = 1 (:29-31)
   fetch(0);
@@ -329,11 +416,11 @@ __device__ __forceinline__ void column_sweep(const AdiArgs& g, int lb, double* l
     double* dy = buf + L::TILE;
 #pragma unroll
     for (int m = 0; m < kCW; ++m) buf[m * W + t] = pre[m];
-    if (t < 2 * kCW) buf[xr * W + NT + (t & 1)] = prex;
-    commit_pair(dy, t, prey);
-    __syncthreads();
+    if (t < 2 * kCW) buf[xr * W + NT + (t & 1)] = extra;
+    commit_pair(dy, t, extra);
+    lds_barrier();
     if (c + 1 < nchunks) fetch(c + 1);
-    forward_chunk(q, f, dy, n, j0, live, g.qs, g.ld, lb + t, [&](int jj) {
+    forward_chunk(q, f, dy, j0, qs, [&](int jj) {
       const double* r = buf + jj * W + t;
       return (g.nd * r[0] + g.c1 * r[1]) - g.f * r[2];
     });
@@ -344,29 +431,22 @@ __device__ __forceinline__ void column_sweep(const AdiArgs& g, int lb, double* l
   row_of(g.v, 0, g.ldv)[i] = 1.0;
   row_of(g.v, n - 1, g.ldv)[i] = 1.0;
   double x = 1.0;
-  double qn[kCW];
   const ConstTab ptab = as_const(g.cols.p);
-  q_chunk(g.qs, g.ld, lb + t, 1 + (nchunks - 1) * kCW, qn);
-  auto chunk = [&](int c, auto partial) {
+  back_chunks(nchunks, qs, [&](int c, const double (&qc)[kCW], auto partial) {
     const int j0 = 1 + c * kCW;
-    double qc[kCW], pj[kCW];
-#pragma unroll
-    for (int kk = 0; kk < kCW; ++kk) qc[kk] = qn[kk];
-    table_chunk(ptab, n, j0, pj);
-    if (c > 0) q_chunk(g.qs, g.ld, lb + t, j0 - kCW, qn);
+    double pj[kCW];
+    table_chunk(ptab, j0, pj);
     back_chunk<decltype(partial)::value>(x, qc, pj, n, j0, [&](int kk, double xv) {
       const int j = j0 + kk;
       row_of(g.v, decltype(partial)::value && j > n - 2 ? j + 1 : j, g.ldv)[i] = xv;
     });
-  };
-  chunk(nchunks - 1, std::true_type{});
-  for (int c = nchunks - 2; c >= 0; --c) chunk(c, std::false_type{});
+  });
 }
 
 // Row sweep (adi_numpy.py:36-46) of the lines lb+1 .. lb+NT: row i of v
 // solved along the columns into row i of u.
-template <int NT>
-__device__ __forceinline__ void row_sweep(const AdiArgs& g, int lb, double* lds) {
+template <int NT, bool QLDS>
+__device__ __forceinline__ void row_sweep(const AdiArgs& g, int lb, double* lds, double* qlds) {
   using L = AdiLds<NT>;
   constexpr int P = kCW + 1;    // LDS row pitch (odd: lanes hit different banks)
   constexpr int RS = NT / kCW;  // tile rows one pass of the workgroup covers
@@ -376,18 +456,23 @@ __device__ __forceinline__ void row_sweep(const AdiArgs& g, int lb, double* lds)
   const bool live = i <= n - 2;
   const int nchunks = (n - 2 + kCW - 1) / kCW;
   const Fwd f = forward_consts(g.rows, g.d);
+  QSlots qs;
+  if constexpr (QLDS)
+    qs = {qlds + t, NT};
+  else
+    qs = {g.qs + lb + t, g.ld};
   // Tile of columns j0 .. j0+kCW-1 of v, rows lb .. lb+NT+1, tile[row][col]:
   // kCW consecutive threads read one row's run (thread t: column j0 + t%kCW
   // of rows lb + t/kCW + m*RS), threads 0..31 also the last two rows; rows
   // and columns clamped into the grid as in the column sweep.
   const int tj = t % kCW, tr = t / kCW;
-  double pre[kCW], prex = 0.0, prey = 0.0;
+  double pre[kCW], extra;
   auto fetch = [&](int c) {
     const int colj = min(1 + c * kCW + tj, n - 2);
 #pragma unroll
     for (int m = 0; m < kCW; ++m) pre[m] = row_of(g.v, min(lb + tr + m * RS, n - 1), g.ldv)[colj];
-    if (t < 2 * kCW) prex = row_of(g.v, min(lb + NT + tr, n - 1), g.ldv)[colj];
-    else if (t < 4 * kCW) prey = fetch_pair(g.rows, t, n, 1 + c * kCW);
+    const double* xa = row_of(g.v, min(lb + NT + (tr & 1), n - 1), g.ldv) + colj;
+    extra = *(t < 2 * kCW ? xa : pair_addr(g.rows, t, 1 + c * kCW));
   };
   double q = 1.0;  // q[0] = u[i, 0] = 1 (:37-39)
   fetch(0);
@@ -397,72 +482,71 @@ __device__ __forceinline__ void row_sweep(const AdiArgs& g, int lb, double* lds)
     double* dy = buf + L::TILE;
 #pragma unroll
     for (int m = 0; m < kCW; ++m) buf[(tr + m * RS) * P + tj] = pre[m];
-    if (t < 2 * kCW) buf[(NT + tr) * P + tj] = prex;
-    commit_pair(dy, t, prey);
-    __syncthreads();
+    if (t < 2 * kCW) buf[(NT + tr) * P + tj] = extra;
+    commit_pair(dy, t, extra);
+    lds_barrier();
     if (c + 1 < nchunks) fetch(c + 1);
-    forward_chunk(q, f, dy, n, j0, live, g.qs, g.ld, lb + t, [&](int jj) {
+    forward_chunk(q, f, dy, j0, qs, [&](int jj) {
       const double* r = buf + t * P + jj;
       return (g.na * r[0] + g.c2 * r[P]) - g.c * r[2 * P];
     });
   }
-  __syncthreads();  // the tiles are reused for the output below
+  lds_barrier();  // the tiles are reused for the output below
   // back substitution (:44-46) with u[i, n-1] = 1; u[i, 0] = 1 (:37)
   if (live) {
     g.u[static_cast<int64_t>(i) * n] = 1.0;
     g.u[static_cast<int64_t>(i) * n + n - 1] = 1.0;
   }
   double x = 1.0;
-  double qn[kCW];
   const ConstTab ptab = as_const(g.rows.p);
-  q_chunk(g.qs, g.ld, lb + t, 1 + (nchunks - 1) * kCW, qn);
-  auto chunk = [&](int c, auto partial) {
+  const bool all_rows = lb + NT <= n - 2;  // every line of the workgroup is live
+  back_chunks(nchunks, qs, [&](int c, const double (&qc)[kCW], auto partial) {
     const int j0 = 1 + c * kCW;
-    double qc[kCW], pj[kCW];
-#pragma unroll
-    for (int kk = 0; kk < kCW; ++kk) qc[kk] = qn[kk];
-    table_chunk(ptab, n, j0, pj);
-    if (c > 0) q_chunk(g.qs, g.ld, lb + t, j0 - kCW, qn);
+    double pj[kCW];
+    table_chunk(ptab, j0, pj);
     double* ob = lds + (c & 1) * L::BUF;  // ob[line][column - j0]
     back_chunk<decltype(partial)::value>(x, qc, pj, n, j0,
                                          [&](int kk, double xv) { ob[t * P + kk] = xv; });
-    __syncthreads();
+    lds_barrier();
     // rows lb+1 .. lb+NT, columns j0 .. j0+kCW-1 of u, kCW threads per row run
     const int colj = j0 + tj;
-    if (colj <= n - 2) {
+    double* urow = g.u + static_cast<int64_t>(lb + 1 + tr) * n + colj;
+    const int64_t rstep = static_cast<int64_t>(RS) * n;
+    if (!decltype(partial)::value && all_rows) {
 #pragma unroll
-      for (int m = 0; m < kCW; ++m) {
-        const int rr = tr + m * RS;
-        if (lb + 1 + rr <= n - 2)
-          g.u[static_cast<int64_t>(lb + 1 + rr) * n + colj] = ob[rr * P + tj];
-      }
+      for (int m = 0; m < kCW; ++m) urow[m * rstep] = ob[(tr + m * RS) * P + tj];
+    } else if (colj <= n - 2) {
+#pragma unroll
+      for (int m = 0; m < kCW; ++m)
+        if (lb + 1 + tr + m * RS <= n - 2) urow[m * rstep] = ob[(tr + m * RS) * P + tj];
     }
-  };
-  chunk(nchunks - 1, std::true_type{});
-  for (int c = nchunks - 2; c >= 0; --c) chunk(c, std::false_type{});
+  });
 }
 
 template <int NT>
 __global__ __launch_bounds__(NT) void adi_fused(AdiArgs g) {
   __shared__ __align__(16) double lds[AdiLds<NT>::ELEMS];
   for (int s = 0; s < g.tsteps; ++s) {
-    column_sweep<NT>(g, 0, lds);
+    column_sweep<NT, false>(g, 0, lds, nullptr);
     __syncthreads();
-    row_sweep<NT>(g, 0, lds);
+    row_sweep<NT, false>(g, 0, lds, nullptr);
     __syncthreads();
   }
 }
 
-template <int NT>
+// One half step over ceil(lines / NT) workgroups; QLDS: q kept in LDS.
+template <int NT, bool QLDS>
 __global__ __launch_bounds__(NT) void adi_columns(AdiArgs g) {
   __shared__ __align__(16) double lds[AdiLds<NT>::ELEMS];
-  column_sweep<NT>(g, static_cast<int>(blockIdx.x) * NT, lds);
+  __shared__ double qlds[QLDS ? kQRows * NT : 1];
+  column_sweep<NT, QLDS>(g, static_cast<int>(blockIdx.x) * NT, lds, qlds);
 }
 
-template <int NT>
+template <int NT, bool QLDS>
 __global__ __launch_bounds__(NT) void adi_rows(AdiArgs g) {
   __shared__ __align__(16) double lds[AdiLds<NT>::ELEMS];
-  row_sweep<NT>(g, static_cast<int>(blockIdx.x) * NT, lds);
+  __shared__ double qlds[QLDS ? kQRows * NT : 1];
+  row_sweep<NT, QLDS>(g, static_cast<int>(blockIdx.x) * NT, lds, qlds);
 }
 
 // Self-check of the short division: random (n, den) pairs, half of them with
@@ -505,10 +589,10 @@ __global__ __launch_bounds__(256) void adi_division_check(uint64_t seed, int64_t
       n = __builtin_fma(den, q0, den * half);
     }
     if (!den_in_range(den)) continue;
-    double lo, hi;
-    numerator_bounds(fabs(den), fabs(den), lo, hi);
-    if (!unscaled_division(n, lo, hi)) continue;
-    const double q = short_division(n, den, division_reciprocal(den));
+    double lo, hi, m;
+    quotient_bounds(fabs(den), lo, hi);
+    const double q = short_division(n, den, division_reciprocal(den), m);
+    if (!unscaled_division(m, lo, hi)) continue;
     ++checked;
     const double ieee = n / den;
     wrong += __double_as_longlong(q) != __double_as_longlong(ieee) ? 1 : 0;
@@ -517,36 +601,52 @@ __global__ __launch_bounds__(256) void adi_division_check(uint64_t seed, int64_t
   atomicAdd(&counts[1], wrong);
 }
 
-constexpr int kSplitNT = 256;
-std::atomic<int> g_adi_path{0};  // 0: by size, 1: one launch, 2: a launch per half step
+// 0: the default (3), 1: every time step in one single-workgroup launch, 2:
+// a launch per half step over 256-line workgroups, 3: the same over one-wave
+// workgroups.  Measured on the MI355X, one wave per CU is the fastest: a
+// line's chain leaves the SIMD idle most of the time, and waves sharing a CU
+// queue behind each other's memory instructions.
+std::atomic<int> g_adi_path{0};
+
+template <int NT, bool QLDS>
+int launch_halves(const AdiArgs& g, int nb, int64_t tsteps, hipStream_t stream) {
+  for (int64_t s = 0; s < tsteps; ++s) {
+    hipLaunchKernelGGL((adi_columns<NT, QLDS>), dim3(nb), dim3(NT), 0, stream, g);
+    NPBX_LAUNCHED("adi_columns");
+    hipLaunchKernelGGL((adi_rows<NT, QLDS>), dim3(nb), dim3(NT), 0, stream, g);
+    NPBX_LAUNCHED("adi_rows");
+  }
+  return NPBX_OK;
+}
 
 int run_adi(double* u, int64_t n64, int64_t tsteps, hipStream_t stream) {
   const int n = static_cast<int>(n64);
   const int lines = n - 2;
   int path = g_adi_path.load();
-  if (path == 0) path = lines <= 256 ? 1 : 2;
-  NPBX_REQUIRE(path == 2 || lines <= 512,
+  if (path == 0) path = 3;
+  NPBX_REQUIRE(path != 1 || lines <= 512,
                "adi: one-launch path holds at most 512 lines (N=%d)", n);
-  const int nt = path == 1 ? (lines <= 256 ? 256 : 512) : kSplitNT;
+  const int nt = path == 1 ? (lines <= 256 ? 256 : 512) : path == 3 ? 64 : 256;
   const int nb = static_cast<int>(cdiv(lines, nt));
   const int64_t ld = static_cast<int64_t>(nb) * nt;
   const int64_t ldv = ld + 1;
   const int64_t qrows = cdiv(lines, kCW) * kCW + 1;
-  const int64_t vel = (n64 + 16) * ldv, qel = qrows * ld, tel = 8 * n64;
+  const int64_t tl = qrows + 1;  // table length: every chunk's block
+  const int64_t vel = (n64 + 16) * ldv, qel = qrows * ld, tel = 8 * tl;
   void* ws = nullptr;
   NPBX_TRY(workspace(static_cast<size_t>(vel + qel + tel) * sizeof(double), &ws));
   double* v = static_cast<double*>(ws);
   double* qs = v + vel;
   double* tab = qs + qel;
   const AdiCoef k = adi_coef(tsteps, n64);
-  hipLaunchKernelGGL(adi_tables, dim3(1), dim3(64), 0, stream, k, n, tab);
+  hipLaunchKernelGGL(adi_tables, dim3(1), dim3(64), 0, stream, k, n, static_cast<int>(tl), tab);
   NPBX_LAUNCHED("adi_tables");
   AdiArgs g;
   g.u = u;
   g.v = v;
   g.qs = qs;
-  g.cols = {tab, tab + n, tab + 2 * n, tab + 3 * n};
-  g.rows = {tab + 4 * n, tab + 5 * n, tab + 6 * n, tab + 7 * n};
+  g.cols = {tab, tab + tl, tab + 2 * tl, tab + 3 * tl};
+  g.rows = {tab + 4 * tl, tab + 5 * tl, tab + 6 * tl, tab + 7 * tl};
   g.nd = -k.d;
   g.c1 = 1.0 + 2.0 * k.d;
   g.f = k.f;
@@ -567,13 +667,9 @@ int run_adi(double* u, int64_t n64, int64_t tsteps, hipStream_t stream) {
     NPBX_LAUNCHED("adi_fused");
     return NPBX_OK;
   }
-  for (int64_t s = 0; s < tsteps; ++s) {
-    hipLaunchKernelGGL(adi_columns<kSplitNT>, dim3(nb), dim3(kSplitNT), 0, stream, g);
-    NPBX_LAUNCHED("adi_columns");
-    hipLaunchKernelGGL(adi_rows<kSplitNT>, dim3(nb), dim3(kSplitNT), 0, stream, g);
-    NPBX_LAUNCHED("adi_rows");
-  }
-  return NPBX_OK;
+  if (nt == 256) return launch_halves<256, false>(g, nb, tsteps, stream);
+  if (qrows <= kQRows) return launch_halves<64, true>(g, nb, tsteps, stream);
+  return launch_halves<64, false>(g, nb, tsteps, stream);
 }
 
 }  // namespace

@@ -1421,7 +1421,7 @@ extern "C" int npbx_set_tuning(const char* key, int64_t value) {
     return NPBX_OK;
   }
   if (std::strcmp(key, "adi.path") == 0) {
-    NPBX_REQUIRE(value >= 0 && value <= 2, "set_tuning: adi.path must be 0, 1 or 2");
+    NPBX_REQUIRE(value >= 0 && value <= 3, "set_tuning: adi.path must be 0, 1, 2 or 3");
     set_adi_path(static_cast<int>(value));
     return NPBX_OK;
   }

@@ -3,10 +3,12 @@
 The HIP path performs, per element, the reference's operations in its order
 (right-hand side, forward elimination with IEEE division, back substitution),
 so every result must be bitwise the oracle's.  Sizes cover one line, lines
-that end inside and on a 16-step chunk, one and several workgroups (256 lines
-each), both launch paths (all time steps in one launch up to 512 lines; one
-launch per half step), the first time step's original boundary columns, and
-the reference's own initialisation at its S and paper presets.
+that end inside and on a 16-step chunk, one and several workgroups, every
+launch path (all time steps in one launch up to 512 lines; one launch per
+half step over 256-line workgroups, or over one-wave workgroups with q in
+LDS up to 272 steps and in device memory beyond), the first time step's
+original boundary columns, and the reference's own initialisation at its S
+and paper presets.
 """
 
 import numpy as np
@@ -55,7 +57,7 @@ CASES = [(3, 1), (4, 2), (5, 3), (17, 2), (18, 1), (19, 2), (33, 3), (100, 4), (
 
 
 @pytest.mark.parametrize("n,tsteps", CASES)
-@pytest.mark.parametrize("path", [0, 1, 2])
+@pytest.mark.parametrize("path", [0, 1, 2, 3])
 def test_adi_fp64_bitwise(st, adi_path, n, tsteps, path):
     if path == 1 and n - 2 > 512:
         pytest.skip("the one-launch path holds at most 512 lines")
