@@ -320,6 +320,26 @@ def kernel_suite(st, dev, torch, clock, reps, copy_only=False):
             "glups": round(I * J * K * reps / ev / 1e9, 2), "avg_us": round(ev / reps * 1e6, 3),
             "hbm_frac": round(byts * reps / ev / 1e9 / HBM_PEAK_GBS, 4)}
         del d
+
+    # ADI (polybench adi, FP64) at the reference's paper and L presets, on the
+    # reference initialisation restored before each call (its values grow
+    # from call to call).  Bound: the serial division chain of each line, not
+    # HBM (a call moves a few MB); a "lup" is one point of one half step.
+    from npbench_amd.benchmarks.polybench.adi import adi as adi_init
+    for preset, T, N in (("paper", 100, 200), ("L", 50, 500)):
+        u0 = dev.to_device(adi_init.initialize(N), torch.float64)
+        u = u0.clone()
+
+        def ad():
+            u.copy_(u0)
+            st.adi(u, T, stream=clock.stream)
+        ad()
+        _, ev = clock.run(ad, reps)
+        res[f"adi_{preset}_N{N}_T{T}_f64"] = {
+            "ms_per_call": round(ev / reps * 1e3, 4),
+            "glups": round(2 * T * (N - 2) ** 2 * reps / ev / 1e9, 4),
+            "bound": "latency (one serial recurrence per line)"}
+        del u0, u
     return res
 
 
@@ -337,6 +357,20 @@ def cpu_rates():
         f(A, B, sweeps)
         out[kind] = (n - 2) ** len(shape) * sweeps / (time.perf_counter() - t0) / 1e9
     return out
+
+
+def cpu_adi_ms():
+    """The NumPy oracle's ADI at the reference's paper preset (TSTEPS=100,
+    N=200; 1 core): milliseconds per call, the median of 3."""
+    from oracle import stencils as ref
+    from npbench_amd.benchmarks.polybench.adi import adi as adi_init
+    times = []
+    for _ in range(3):
+        u = adi_init.initialize(200)
+        t0 = time.perf_counter()
+        ref.adi_kernel(100, 200, u)
+        times.append(time.perf_counter() - t0)
+    return round(statistics.median(times) * 1e3, 1)
 
 
 def cpu_baseline(n, sweeps):
@@ -453,6 +487,10 @@ def main():
                                   cpu_rates=cpu_rates() if args.cpu_sweeps > 0 else None)
     if not args.no_kernels:
         line["kernels"] = kernel_suite(st, dev, torch, clock, 5)
+        if args.cpu_sweeps > 0:
+            adi = line["kernels"]["adi_paper_N200_T100_f64"]
+            adi["cpu_oracle_ms"] = cpu_adi_ms()
+            adi["speedup_vs_cpu_oracle"] = round(adi["cpu_oracle_ms"] / adi["ms_per_call"], 1)
     if args.cpu_sweeps > 0:
         line["cpu_baseline"] = cpu_baseline(HEAT_N, args.cpu_sweeps)
     print(json.dumps(line), flush=True)
