@@ -223,6 +223,12 @@ int npbx_vadv(int field_dtype, void *utens_stage, const void *u_stage,
               const void *wcon, const void *u_pos, const void *utens,
               double dtr_stage, int64_t I, int64_t J, int64_t K, void *stream);
 
+/* ---- 1-D three-point Jacobi (jacobi_1d_numpy.py:4-8) -----------------------
+ * A, B: (N,) FP64 (dtype NPBX_F64), updated in place by TSTEPS - 1 ping-pong
+ * iterations; bitwise the NumPy result.  Boundaries are never written. */
+int npbx_jacobi1d(int dtype, void *A, void *B, int64_t N, int64_t tsteps,
+                  void *stream);
+
 /* ---- alternating-direction implicit solver (adi_numpy.py:6-52) -------------
  * u: (N, N) FP64 (dtype NPBX_F64), updated in place by TSTEPS time steps of
  * the reference, each a column sweep into library scratch and a row sweep

@@ -3,10 +3,10 @@
 Two layers:
 
 * resident operations (``jacobi2d_sweeps``, ``heat3d_sweeps``, ``hdiff``,
-  ``vadv``, ``adi``) take device tensors already in their storage dtype (float64,
+  ``vadv``, ``adi``, ``jacobi1d``) take device tensors already in their storage dtype (float64,
   float32 or bfloat16) and enqueue HIP kernels on a stream without waiting;
 * drop-in operations (``jacobi_2d``, ``heat_3d``, ``hdiff_dropin``,
-  ``vadv_dropin``, ``adi_dropin``) take the reference's float64 arrays -- device tensors
+  ``vadv_dropin``, ``adi_dropin``, ``jacobi_1d``) take the reference's float64 arrays -- device tensors
   produced by the framework's copy function, or host NumPy arrays, which are
   copied to the device, processed and copied back in place -- and a
   ``precision`` that picks the storage form.  FP32/BF16 forms convert to the
@@ -82,6 +82,17 @@ def adi(u, tsteps, stream=None):
     if u.dim() != 2 or u.shape[0] != u.shape[1]:
         raise ValueError("adi: u must be a square (N, N) array")
     nat.call("npbx_adi", dev.dtype_code(u), dev.ptr(u), u.shape[0], int(tsteps),
+             dev.stream_handle(stream))
+
+
+def jacobi1d(A, B, tsteps, stream=None):
+    """jacobi_1d_numpy.kernel's TSTEPS - 1 iterations on float64 (N,) device
+    tensors, in place."""
+    dev.check_device_array("A", A, dtype=torch.float64)
+    dev.check_device_array("B", B, shape=tuple(A.shape), dtype=torch.float64)
+    if A.dim() != 1:
+        raise ValueError("jacobi1d: A and B must be 1-D")
+    nat.call("npbx_jacobi1d", dev.dtype_code(A), dev.ptr(A), dev.ptr(B), A.shape[0], int(tsteps),
              dev.stream_handle(stream))
 
 
@@ -183,3 +194,9 @@ def adi_dropin(TSTEPS, N, u):
     with _Staged("fp64", [u], outputs=(0,)) as (w,):
         adi(w, TSTEPS)
     return u
+
+
+def jacobi_1d(TSTEPS, A, B):
+    """jacobi_1d_numpy.kernel(TSTEPS, A, B): A and B updated in place."""
+    with _Staged("fp64", [A, B], outputs=(0, 1)) as (a, b):
+        jacobi1d(a, b, TSTEPS)

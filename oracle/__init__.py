@@ -36,4 +36,5 @@ from .stencils import (  # noqa: F401
     jacobi_sweeps,
     heat_sweeps,
     adi_kernel,
+    jacobi_1d_kernel,
 )

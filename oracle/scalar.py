@@ -8,8 +8,8 @@ operand in ``oracle.stencils`` shows up as a bit difference.  Each function
 returns new arrays and leaves its inputs alone.
 
 References: jacobi_2d_numpy.py:4-10, heat_3d_numpy.py:4-20,
-hdiff_numpy.py:5-29, vadv_numpy.py:4-78, adi_numpy.py:6-52
-(npbench/benchmarks/...).
+hdiff_numpy.py:5-29, vadv_numpy.py:4-78, adi_numpy.py:6-52,
+jacobi_1d_numpy.py:4-8 (npbench/benchmarks/...).
 """
 
 import numpy as np
@@ -214,3 +214,16 @@ def adi(TSTEPS, N, u):
             for j in range(n - 2, 0, -1):
                 u[i][j] = p[j] * u[i][j + 1] + q[j]
     return np.array(u)
+
+
+def jacobi_1d(TSTEPS, A, B):
+    """jacobi_1d_numpy.kernel point by point; returns new (A, B)."""
+    a = [float(x) for x in np.asarray(A)]
+    b = [float(x) for x in np.asarray(B)]
+    n = len(a)
+    for _ in range(1, TSTEPS):
+        for src, dst in ((a, b), (b, a)):
+            new = [src[i - 1] + src[i] + src[i + 1] for i in range(1, n - 1)]
+            for i in range(1, n - 1):
+                dst[i] = 0.33333 * new[i - 1]
+    return np.array(a), np.array(b)

@@ -10,6 +10,7 @@ Reference files (paths relative to the npbench source tree):
   hdiff             npbench/benchmarks/weather_stencils/hdiff/hdiff_numpy.py:5-29
   vadv              npbench/benchmarks/weather_stencils/vadv/vadv_numpy.py:4-78
   adi_kernel        npbench/benchmarks/polybench/adi/adi_numpy.py:6-52
+  jacobi_1d_kernel  npbench/benchmarks/polybench/jacobi_1d/jacobi_1d_numpy.py:4-8
 """
 
 import numpy as np
@@ -312,3 +313,22 @@ def adi_kernel(TSTEPS, N, u):
         adi_column_sweep(u, v, (a, b, c, d, f))
         adi_row_sweep(v, u, (a, c, d, e, f))
     return u
+
+
+# --------------------------------------------------------------------------
+# 1-D three-point Jacobi
+# --------------------------------------------------------------------------
+def _jacobi1_sweep(src, dst):
+    """dst[1:-1] = 0.33333 * ((left + centre) + right) (jacobi_1d_numpy.py:7-8);
+    dst's two ends are never written."""
+    acc = src[:-2] + src[1:-1]
+    acc = acc + src[2:]
+    dst[1:-1] = 0.33333 * acc
+
+
+def jacobi_1d_kernel(TSTEPS, A, B):
+    """jacobi_1d_numpy.kernel: TSTEPS - 1 ping-pong iterations in place."""
+    _f64(A, B)
+    for _ in range(1, TSTEPS):
+        _jacobi1_sweep(A, B)
+        _jacobi1_sweep(B, A)

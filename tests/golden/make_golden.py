@@ -19,6 +19,7 @@ Cases (all FP64):
               (wcon in [-1,1)), and the K=2 edge case
   adi         N=9  TSTEPS=3 reference init (fromfunction); N=6 TSTEPS=2
               seeded uniform
+  jacobi_1d   N=10 TSTEPS=4 reference init; N=9 TSTEPS=5 seeded uniform
 """
 
 import os
@@ -36,6 +37,7 @@ from npbench_amd.benchmarks.polybench.heat_3d import heat_3d  # noqa: E402
 from npbench_amd.benchmarks.weather_stencils.hdiff import hdiff  # noqa: E402
 from npbench_amd.benchmarks.weather_stencils.vadv import vadv  # noqa: E402
 from npbench_amd.benchmarks.polybench.adi import adi  # noqa: E402
+from npbench_amd.benchmarks.polybench.jacobi_1d import jacobi_1d  # noqa: E402
 
 
 def save(name, **arrays):
@@ -86,6 +88,13 @@ def main():
                               ("uniform", 6, 2, np.random.default_rng(5).random((6, 6)))):
         out = scalar.adi(tsteps, n, u)
         save(f"adi_{tag}", TSTEPS=tsteps, N=n, u=u, out=out)
+
+    # jacobi 1-D
+    rng = np.random.default_rng(13)
+    for tag, tsteps, (A, B) in (("ref", 4, jacobi_1d.initialize(10)),
+                                ("uniform", 5, (rng.random(9), rng.random(9)))):
+        oa, ob = scalar.jacobi_1d(tsteps, A, B)
+        save(f"jacobi_1d_{tag}", TSTEPS=tsteps, A=A, B=B, A_out=oa, B_out=ob)
 
 
 if __name__ == "__main__":

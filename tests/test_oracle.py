@@ -27,6 +27,8 @@ fails at least one check:
              test; the reference's coefficients (b = 1 + mul2); untouched
              rows 0 and N-1 and the boundary columns set to 1; brute force
              and golden vectors.
+  jacobi_1d  discrete Fourier eigenmode with eigenvalue 0.33333 (1 + 2 cos a);
+             untouched ends; linearity; brute force and golden vectors.
 
 The last section plants mistakes in the oracle (monkeypatched copies) and
 checks the pins above catch them.
@@ -438,6 +440,58 @@ def test_adi_bruteforce_bitwise(n, tsteps):
     assert np.array_equal(S.adi_kernel(tsteps, n, u.copy()), scalar.adi(tsteps, n, u))
 
 
+# ------------------------------------------------------------------ jacobi 1-D
+def jacobi1_mode_err(kernel=S.jacobi_1d_kernel, n=41, k=3, tsteps=5):
+    i = np.arange(n, dtype=np.float64)
+    al = k * np.pi / (n - 1)
+    mode = np.sin(al * i)  # zero at both ends
+    lam = 0.33333 * (1.0 + 2.0 * np.cos(al))
+    A, B = mode.copy(), mode.copy()
+    kernel(tsteps, A, B)
+    s = 2 * (tsteps - 1)
+    return max(np.max(np.abs(A - lam ** s * mode)), np.max(np.abs(B - lam ** (s - 1) * mode)))
+
+
+@pytest.mark.parametrize("k", [1, 4, 9])
+def test_jacobi1d_eigenmode(k):
+    assert jacobi1_mode_err(k=k) < 1e-13
+
+
+def test_jacobi1d_ends_untouched_and_linear():
+    rng = np.random.default_rng(4)
+    A, B, C, D = (rng.random(30) for _ in range(4))
+    a, b = A.copy(), B.copy()
+    S.jacobi_1d_kernel(6, a, b)
+    assert a[0] == A[0] and a[-1] == A[-1] and b[0] == B[0] and b[-1] == B[-1]
+    c, d = C.copy(), D.copy()
+    S.jacobi_1d_kernel(6, c, d)
+    e, f = A + 2.0 * C, B + 2.0 * D
+    S.jacobi_1d_kernel(6, e, f)
+    assert np.allclose(e, a + 2.0 * c, rtol=1e-13, atol=1e-13)
+    assert np.allclose(f, b + 2.0 * d, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("n,tsteps", [(3, 3), (4, 2), (12, 6), (40, 9)])
+def test_jacobi1d_bruteforce_bitwise(n, tsteps):
+    rng = np.random.default_rng(n + tsteps)
+    A, B = rng.random(n), rng.random(n)
+    a, b = A.copy(), B.copy()
+    S.jacobi_1d_kernel(tsteps, a, b)
+    ra, rb = scalar.jacobi_1d(tsteps, A, B)
+    assert np.array_equal(a, ra) and np.array_equal(b, rb)
+
+
+def test_planted_jacobi1d_mistakes_are_caught(monkeypatch):
+    """A dropped neighbour and a shifted neighbour both break the eigenmode."""
+    for mistake in ("drop", "shift"):
+        def sweep(src, dst, mistake=mistake):
+            right = src[2:] if mistake == "drop" else src[1:-1]
+            acc = src[:-2] + src[1:-1] if mistake != "drop" else src[1:-1] + src[1:-1]
+            dst[1:-1] = 0.33333 * (acc + right)
+        monkeypatch.setattr(S, "_jacobi1_sweep", sweep)
+        assert jacobi1_mode_err() > 1e-6, mistake
+
+
 # ------------------------------------------------------------------ golden
 def _golden(name):
     with np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False) as z:
@@ -471,6 +525,14 @@ def test_golden_hdiff(name):
     out = g["out_field"].copy()
     S.hdiff_iterated(g["in_field"], out, g["coeff"], int(g.get("nsteps", 1)))
     assert np.array_equal(out, g["out"])
+
+
+@pytest.mark.parametrize("name", ["jacobi_1d_ref", "jacobi_1d_uniform"])
+def test_golden_jacobi_1d(name):
+    g = _golden(name)
+    a, b = g["A"].copy(), g["B"].copy()
+    S.jacobi_1d_kernel(int(g["TSTEPS"]), a, b)
+    assert np.array_equal(a, g["A_out"]) and np.array_equal(b, g["B_out"])
 
 
 @pytest.mark.parametrize("name", ["adi_ref", "adi_uniform"])
