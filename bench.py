@@ -340,6 +340,46 @@ def kernel_suite(st, dev, torch, clock, reps, copy_only=False):
             "glups": round(2 * T * (N - 2) ** 2 * reps / ev / 1e9, 4),
             "bound": "latency (one serial recurrence per line)"}
         del u0, u
+
+    # 1-D Jacobi (polybench jacobi_1d, FP64) at the paper preset: 7998 sweeps
+    # of 32000 points in rounds of 64 sweeps per launch
+    from npbench_amd.benchmarks.polybench.jacobi_1d import jacobi_1d as j1_init
+    T1, N1 = 4000, 32000
+    a0, b0 = (dev.to_device(x, torch.float64) for x in j1_init.initialize(N1))
+    a1, b1 = a0.clone(), b0.clone()
+
+    def j1():
+        a1.copy_(a0)
+        b1.copy_(b0)
+        st.jacobi1d(a1, b1, T1, stream=clock.stream)
+    j1()
+    _, ev = clock.run(j1, reps)
+    res[f"jacobi_1d_paper_N{N1}_T{T1}_f64"] = {
+        "ms_per_call": round(ev / reps * 1e3, 4),
+        "glups": round(2 * (T1 - 1) * (N1 - 2) * reps / ev / 1e9, 2),
+        "bound": "latency (rounds of 64 dependent sweeps, a launch each)"}
+    del a0, b0, a1, b1
+
+    # 2-D FDTD (polybench fdtd_2d, FP64) at the L and paper presets: two
+    # launches per time step; 9 field passes per step (ex, ey, hz read and
+    # ex, ey written by the first launch; ex, ey, hz read and hz written by
+    # the second), the fields resident in the 256 MB cache
+    from npbench_amd.benchmarks.polybench.fdtd_2d import fdtd_2d as fd_init
+    for preset, TF, NX, NY in (("L", 150, 800, 900), ("paper", 500, 1000, 1200)):
+        f0 = [dev.to_device(x, torch.float64) for x in fd_init.initialize(TF, NX, NY)]
+        f1 = [x.clone() for x in f0]
+
+        def fd():
+            for x, y in zip(f1, f0):
+                x.copy_(y)
+            st.fdtd2d(*f1, TF, stream=clock.stream)
+        fd()
+        _, ev = clock.run(fd, reps)
+        res[f"fdtd_2d_{preset}_{NX}x{NY}_T{TF}_f64"] = {
+            "ms_per_call": round(ev / reps * 1e3, 4),
+            "glups": round(TF * NX * NY * reps / ev / 1e9, 2),
+            "field_pass_GBps": round(9 * 8 * TF * NX * NY * reps / ev / 1e9, 1)}
+        del f0, f1
     return res
 
 
@@ -357,6 +397,34 @@ def cpu_rates():
         f(A, B, sweeps)
         out[kind] = (n - 2) ** len(shape) * sweeps / (time.perf_counter() - t0) / 1e9
     return out
+
+
+def cpu_jacobi1d_ms():
+    """The NumPy oracle's jacobi_1d at the paper preset (TSTEPS=4000,
+    N=32000; 1 core): milliseconds per call, the median of 3."""
+    from oracle import stencils as ref
+    from npbench_amd.benchmarks.polybench.jacobi_1d import jacobi_1d as j1_init
+    times = []
+    for _ in range(3):
+        A, B = j1_init.initialize(32000)
+        t0 = time.perf_counter()
+        ref.jacobi_1d_kernel(4000, A, B)
+        times.append(time.perf_counter() - t0)
+    return round(statistics.median(times) * 1e3, 1)
+
+
+def cpu_fdtd2d_ms():
+    """The NumPy oracle's fdtd_2d at the L preset (TMAX=150, 800 x 900; 1
+    core): milliseconds per call, the median of 3."""
+    from oracle import stencils as ref
+    from npbench_amd.benchmarks.polybench.fdtd_2d import fdtd_2d as fd_init
+    times = []
+    for _ in range(3):
+        ex, ey, hz, fict = fd_init.initialize(150, 800, 900)
+        t0 = time.perf_counter()
+        ref.fdtd_2d_kernel(150, ex, ey, hz, fict)
+        times.append(time.perf_counter() - t0)
+    return round(statistics.median(times) * 1e3, 1)
 
 
 def cpu_adi_ms():
@@ -491,6 +559,12 @@ def main():
             adi = line["kernels"]["adi_paper_N200_T100_f64"]
             adi["cpu_oracle_ms"] = cpu_adi_ms()
             adi["speedup_vs_cpu_oracle"] = round(adi["cpu_oracle_ms"] / adi["ms_per_call"], 1)
+            j1 = line["kernels"]["jacobi_1d_paper_N32000_T4000_f64"]
+            j1["cpu_oracle_ms"] = cpu_jacobi1d_ms()
+            j1["speedup_vs_cpu_oracle"] = round(j1["cpu_oracle_ms"] / j1["ms_per_call"], 1)
+            fd = line["kernels"]["fdtd_2d_L_800x900_T150_f64"]
+            fd["cpu_oracle_ms"] = cpu_fdtd2d_ms()
+            fd["speedup_vs_cpu_oracle"] = round(fd["cpu_oracle_ms"] / fd["ms_per_call"], 1)
     if args.cpu_sweeps > 0:
         line["cpu_baseline"] = cpu_baseline(HEAT_N, args.cpu_sweeps)
     print(json.dumps(line), flush=True)

@@ -229,6 +229,12 @@ int npbx_vadv(int field_dtype, void *utens_stage, const void *u_stage,
 int npbx_jacobi1d(int dtype, void *A, void *B, int64_t N, int64_t tsteps,
                   void *stream);
 
+/* ---- 2-D finite-difference time domain (fdtd_2d_numpy.py:4-11) ------------
+ * ex, ey, hz: (NX, NY) FP64 (dtype NPBX_F64), updated in place by TMAX time
+ * steps; fict: (>= TMAX,) FP64, read.  Bitwise the NumPy result. */
+int npbx_fdtd2d(int dtype, void *ex, void *ey, void *hz, const void *fict,
+                int64_t NX, int64_t NY, int64_t tmax, void *stream);
+
 /* ---- alternating-direction implicit solver (adi_numpy.py:6-52) -------------
  * u: (N, N) FP64 (dtype NPBX_F64), updated in place by TSTEPS time steps of
  * the reference, each a column sweep into library scratch and a row sweep

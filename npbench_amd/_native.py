@@ -92,6 +92,7 @@ SIGNATURES = {
     "npbx_vadv": [_int, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _i64, _i64, _i64, _vp],
     "npbx_adi": [_int, _vp, _i64, _i64, _vp],
     "npbx_jacobi1d": [_int, _vp, _vp, _i64, _i64, _vp],
+    "npbx_fdtd2d": [_int, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp],
     "npbx_adi_division_check": [_i64, ctypes.c_uint64, ctypes.POINTER(_i64),
                                 ctypes.POINTER(_i64)],
 }

@@ -3,10 +3,10 @@
 Two layers:
 
 * resident operations (``jacobi2d_sweeps``, ``heat3d_sweeps``, ``hdiff``,
-  ``vadv``, ``adi``, ``jacobi1d``) take device tensors already in their storage dtype (float64,
+  ``vadv``, ``adi``, ``jacobi1d``, ``fdtd2d``) take device tensors already in their storage dtype (float64,
   float32 or bfloat16) and enqueue HIP kernels on a stream without waiting;
 * drop-in operations (``jacobi_2d``, ``heat_3d``, ``hdiff_dropin``,
-  ``vadv_dropin``, ``adi_dropin``, ``jacobi_1d``) take the reference's float64 arrays -- device tensors
+  ``vadv_dropin``, ``adi_dropin``, ``jacobi_1d``, ``fdtd_2d``) take the reference's float64 arrays -- device tensors
   produced by the framework's copy function, or host NumPy arrays, which are
   copied to the device, processed and copied back in place -- and a
   ``precision`` that picks the storage form.  FP32/BF16 forms convert to the
@@ -94,6 +94,21 @@ def jacobi1d(A, B, tsteps, stream=None):
         raise ValueError("jacobi1d: A and B must be 1-D")
     nat.call("npbx_jacobi1d", dev.dtype_code(A), dev.ptr(A), dev.ptr(B), A.shape[0], int(tsteps),
              dev.stream_handle(stream))
+
+
+def fdtd2d(ex, ey, hz, fict, tmax, stream=None):
+    """fdtd_2d_numpy.kernel's TMAX time steps on float64 (NX, NY) device
+    tensors ex, ey, hz (in place) with the float64 source fict."""
+    dev.check_device_array("ex", ex, dtype=torch.float64)
+    if ex.dim() != 2:
+        raise ValueError("fdtd2d: fields must be 2-D (NX, NY)")
+    for name, t in (("ey", ey), ("hz", hz)):
+        dev.check_device_array(name, t, shape=tuple(ex.shape), dtype=torch.float64)
+    dev.check_device_array("fict", fict, dtype=torch.float64)
+    if fict.dim() != 1 or fict.shape[0] < int(tmax):
+        raise ValueError(f"fdtd2d: fict needs at least TMAX={int(tmax)} entries")
+    nat.call("npbx_fdtd2d", dev.dtype_code(ex), dev.ptr(ex), dev.ptr(ey), dev.ptr(hz),
+             dev.ptr(fict), ex.shape[0], ex.shape[1], int(tmax), dev.stream_handle(stream))
 
 
 def cast_into(src, dst, stream=None):
@@ -200,3 +215,10 @@ def jacobi_1d(TSTEPS, A, B):
     """jacobi_1d_numpy.kernel(TSTEPS, A, B): A and B updated in place."""
     with _Staged("fp64", [A, B], outputs=(0, 1)) as (a, b):
         jacobi1d(a, b, TSTEPS)
+
+
+def fdtd_2d(TMAX, ex, ey, hz, _fict_):
+    """fdtd_2d_numpy.kernel(TMAX, ex, ey, hz, _fict_): the fields updated in
+    place."""
+    with _Staged("fp64", [ex, ey, hz, _fict_], outputs=(0, 1, 2)) as (a, b, c, f):
+        fdtd2d(a, b, c, f, TMAX)

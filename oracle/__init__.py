@@ -37,4 +37,5 @@ from .stencils import (  # noqa: F401
     heat_sweeps,
     adi_kernel,
     jacobi_1d_kernel,
+    fdtd_2d_kernel,
 )

@@ -9,7 +9,7 @@ returns new arrays and leaves its inputs alone.
 
 References: jacobi_2d_numpy.py:4-10, heat_3d_numpy.py:4-20,
 hdiff_numpy.py:5-29, vadv_numpy.py:4-78, adi_numpy.py:6-52,
-jacobi_1d_numpy.py:4-8 (npbench/benchmarks/...).
+jacobi_1d_numpy.py:4-8, fdtd_2d_numpy.py:4-11 (npbench/benchmarks/...).
 """
 
 import numpy as np
@@ -227,3 +227,24 @@ def jacobi_1d(TSTEPS, A, B):
             for i in range(1, n - 1):
                 dst[i] = 0.33333 * new[i - 1]
     return np.array(a), np.array(b)
+
+
+def fdtd_2d(TMAX, ex, ey, hz, fict):
+    """fdtd_2d_numpy.kernel point by point; returns new (ex, ey, hz)."""
+    ex = [[float(x) for x in row] for row in np.asarray(ex)]
+    ey = [[float(x) for x in row] for row in np.asarray(ey)]
+    hz = [[float(x) for x in row] for row in np.asarray(hz)]
+    nx, ny = len(ex), len(ex[0])
+    for t in range(TMAX):
+        for j in range(ny):
+            ey[0][j] = float(fict[t])
+        for i in range(1, nx):
+            for j in range(ny):
+                ey[i][j] = ey[i][j] - 0.5 * (hz[i][j] - hz[i - 1][j])
+        for i in range(nx):
+            for j in range(1, ny):
+                ex[i][j] = ex[i][j] - 0.5 * (hz[i][j] - hz[i][j - 1])
+        for i in range(nx - 1):
+            for j in range(ny - 1):
+                hz[i][j] = hz[i][j] - 0.7 * (ex[i][j + 1] - ex[i][j] + ey[i + 1][j] - ey[i][j])
+    return np.array(ex), np.array(ey), np.array(hz)

@@ -11,6 +11,7 @@ Reference files (paths relative to the npbench source tree):
   vadv              npbench/benchmarks/weather_stencils/vadv/vadv_numpy.py:4-78
   adi_kernel        npbench/benchmarks/polybench/adi/adi_numpy.py:6-52
   jacobi_1d_kernel  npbench/benchmarks/polybench/jacobi_1d/jacobi_1d_numpy.py:4-8
+  fdtd_2d_kernel    npbench/benchmarks/polybench/fdtd_2d/fdtd_2d_numpy.py:4-11
 """
 
 import numpy as np
@@ -332,3 +333,25 @@ def jacobi_1d_kernel(TSTEPS, A, B):
     for _ in range(1, TSTEPS):
         _jacobi1_sweep(A, B)
         _jacobi1_sweep(B, A)
+
+
+# --------------------------------------------------------------------------
+# 2-D finite-difference time domain
+# --------------------------------------------------------------------------
+def fdtd_2d_kernel(TMAX, ex, ey, hz, _fict_):
+    """fdtd_2d_numpy.kernel: TMAX time steps, ex, ey, hz updated in place.
+
+    Per step (fdtd_2d_numpy.py:7-11): ey's first row takes fict[t]; ey (rows
+    1..) and ex (columns 1..) lose 0.5 times hz's backward difference along
+    their axis; then hz (all but the last row and column) loses 0.7 times
+    ((ex[i, j+1] - ex[i, j]) + ey[i+1, j]) - ey[i, j], with the new e-fields.
+    """
+    _f64(ex, ey, hz, _fict_)
+    for t in range(TMAX):
+        ey[0, :] = _fict_[t]
+        ey[1:, :] = ey[1:, :] - 0.5 * (hz[1:, :] - hz[:-1, :])
+        ex[:, 1:] = ex[:, 1:] - 0.5 * (hz[:, 1:] - hz[:, :-1])
+        curl = ex[:-1, 1:] - ex[:-1, :-1]
+        curl = curl + ey[1:, :-1]
+        curl = curl - ey[:-1, :-1]
+        hz[:-1, :-1] = hz[:-1, :-1] - 0.7 * curl

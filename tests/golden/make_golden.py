@@ -20,6 +20,7 @@ Cases (all FP64):
   adi         N=9  TSTEPS=3 reference init (fromfunction); N=6 TSTEPS=2
               seeded uniform
   jacobi_1d   N=10 TSTEPS=4 reference init; N=9 TSTEPS=5 seeded uniform
+  fdtd_2d     (NX, NY) = (5, 6) TMAX=3 reference init; (4, 7) TMAX=4 seeded
 """
 
 import os
@@ -38,6 +39,7 @@ from npbench_amd.benchmarks.weather_stencils.hdiff import hdiff  # noqa: E402
 from npbench_amd.benchmarks.weather_stencils.vadv import vadv  # noqa: E402
 from npbench_amd.benchmarks.polybench.adi import adi  # noqa: E402
 from npbench_amd.benchmarks.polybench.jacobi_1d import jacobi_1d  # noqa: E402
+from npbench_amd.benchmarks.polybench.fdtd_2d import fdtd_2d  # noqa: E402
 
 
 def save(name, **arrays):
@@ -95,6 +97,16 @@ def main():
                                 ("uniform", 5, (rng.random(9), rng.random(9)))):
         oa, ob = scalar.jacobi_1d(tsteps, A, B)
         save(f"jacobi_1d_{tag}", TSTEPS=tsteps, A=A, B=B, A_out=oa, B_out=ob)
+
+    # fdtd 2-D
+    rng = np.random.default_rng(17)
+    for tag, tmax, fields in (("ref", 3, fdtd_2d.initialize(3, 5, 6)),
+                              ("uniform", 4, tuple(rng.random(sh) for sh in
+                                                   ((4, 7), (4, 7), (4, 7), (4,))))):
+        ex, ey, hz, fict = fields
+        oex, oey, ohz = scalar.fdtd_2d(tmax, ex, ey, hz, fict)
+        save(f"fdtd_2d_{tag}", TMAX=tmax, ex=ex, ey=ey, hz=hz, fict=fict, ex_out=oex,
+             ey_out=oey, hz_out=ohz)
 
 
 if __name__ == "__main__":

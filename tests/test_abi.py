@@ -60,6 +60,8 @@ def test_version_string():
     (lambda: nat.call("npbx_adi", 0, None, 10, 5, None), "null"),
     (lambda: nat.call("npbx_jacobi1d", 1, None, None, 10, 5, None), "dtype"),
     (lambda: nat.call("npbx_jacobi1d", 0, None, None, 10, 5, None), "distinct"),
+    (lambda: nat.call("npbx_fdtd2d", 1, None, None, None, None, 4, 4, 2, None), "dtype"),
+    (lambda: nat.call("npbx_fdtd2d", 0, None, None, None, None, 4, 4, 2, None), "null"),
 ])
 def test_argument_errors_raise_with_message(call, needle):
     with pytest.raises(nat.NpbxError) as ei:
@@ -77,6 +79,8 @@ def test_empty_work_is_a_no_op():
     assert nat.call("npbx_adi", 0, None, 10, -3, None) == nat.OK   # no time step
     assert nat.call("npbx_jacobi1d", 0, None, None, 2, 5, None) == nat.OK  # no interior
     assert nat.call("npbx_jacobi1d", 0, None, None, 10, 1, None) == nat.OK  # no sweep
+    assert nat.call("npbx_fdtd2d", 0, None, None, None, None, 0, 5, 3, None) == nat.OK
+    assert nat.call("npbx_fdtd2d", 0, None, None, None, None, 4, 5, 0, None) == nat.OK
 
 
 def test_last_error_buffer_is_terminated():

@@ -29,6 +29,11 @@ fails at least one check:
              and golden vectors.
   jacobi_1d  discrete Fourier eigenmode with eigenvalue 0.33333 (1 + 2 cos a);
              untouched ends; linearity; brute force and golden vectors.
+  fdtd_2d    one step's response to an hz impulse, worked out by hand (pins
+             both coefficients, every sign and neighbour direction); constant
+             fields are a fixed point; ey's first row holds the last source
+             value; linearity; fields that vary along i only keep ex at zero
+             and every column equal; brute force and golden vectors.
 
 The last section plants mistakes in the oracle (monkeypatched copies) and
 checks the pins above catch them.
@@ -492,6 +497,105 @@ def test_planted_jacobi1d_mistakes_are_caught(monkeypatch):
         assert jacobi1_mode_err() > 1e-6, mistake
 
 
+# ------------------------------------------------------------------ fdtd 2-D
+def fdtd_impulse(kernel=None):
+    kernel = kernel or S.fdtd_2d_kernel  # looked up per call (planted mistakes patch it)
+    n = 7
+    ex, ey, hz = np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n))
+    hz[3, 3] = 1.0
+    kernel(1, ex, ey, hz, np.zeros(1))
+    return ex, ey, hz
+
+
+def test_fdtd2d_impulse_response():
+    """hz = delta at (3, 3), e = 0, source 0, one step: ey[3,3] = -0.5,
+    ey[4,3] = 0.5, ex[3,3] = -0.5, ex[3,4] = 0.5; then hz[3,3] = 1 - 0.7 * 2,
+    hz[2,3] = hz[3,2] = -0.7 * -0.5, hz[4,3] = hz[3,4] = -0.7 * ... (see below)."""
+    ex, ey, hz = fdtd_impulse()
+    want_ey = np.zeros((7, 7))
+    want_ey[3, 3], want_ey[4, 3] = -0.5, 0.5
+    want_ex = np.zeros((7, 7))
+    want_ex[3, 3], want_ex[3, 4] = -0.5, 0.5
+    assert np.array_equal(ey, want_ey) and np.array_equal(ex, want_ex)
+    # hz[i,j] -= 0.7 ((ex[i,j+1] - ex[i,j]) + ey[i+1,j] - ey[i,j])
+    want_hz = np.zeros((7, 7))
+    want_hz[3, 3] = 1.0 - 0.7 * 2.0        # (0.5 + 0.5) + (0.5 + 0.5)
+    want_hz[2, 3] = -0.7 * -0.5            # ey[3,3] - ey[2,3] enters as +ey[i+1,j]
+    want_hz[3, 2] = -0.7 * -0.5            # ex[3,3] enters as +ex[i,j+1]
+    want_hz[4, 3] = -0.7 * (0.0 - 0.5)     # -ey[4,3]
+    want_hz[3, 4] = -0.7 * (0.0 - 0.5)     # -ex[3,4]
+    assert np.allclose(hz, want_hz, rtol=0, atol=1e-15)
+
+
+def test_fdtd2d_fixed_point_and_source_row():
+    n, tmax = 9, 5
+    ex, ey, hz = np.full((n, n + 2), 0.25), np.full((n, n + 2), 2.0), np.full((n, n + 2), -1.5)
+    fict = np.full(tmax, 2.0)
+    a, b, c = ex.copy(), ey.copy(), hz.copy()
+    S.fdtd_2d_kernel(tmax, a, b, c, fict)
+    assert np.array_equal(a, ex) and np.array_equal(b, ey) and np.array_equal(c, hz)
+    rng = np.random.default_rng(2)
+    a, b, c = (rng.random((6, 8)) for _ in range(3))
+    f = rng.random(tmax)
+    S.fdtd_2d_kernel(tmax, a, b, c, f)
+    assert np.all(b[0] == f[tmax - 1])
+
+
+def test_fdtd2d_linearity_and_plane_waves():
+    rng = np.random.default_rng(3)
+    x = [rng.random((8, 9)) for _ in range(3)] + [rng.random(6)]
+    y = [rng.random((8, 9)) for _ in range(3)] + [rng.random(6)]
+    def run(f):
+        a, b, c = (v.copy() for v in f[:3])
+        S.fdtd_2d_kernel(6, a, b, c, f[3])
+        return a, b, c
+    rx, ry = run(x), run(y)
+    rz = run([u + 3.0 * v for u, v in zip(x, y)])
+    for p, q, r in zip(rx, ry, rz):
+        assert np.allclose(r, p + 3.0 * q, rtol=1e-12, atol=1e-12)
+    # fields that vary along i only: ex stays 0 and all columns evolve alike,
+    # except near the last column (hz's is never updated; that reaches one
+    # column further in per step)
+    col = rng.random((8, 1))
+    ex, ey, hz = np.zeros((8, 20)), np.repeat(col, 20, axis=1), np.repeat(2 * col, 20, axis=1)
+    S.fdtd_2d_kernel(6, ex, ey, hz, rng.random(6))
+    assert np.all(ex[:, :10] == 0.0)
+    assert np.all(ey[:, :10] == ey[:, :1]) and np.all(hz[:, :10] == hz[:, :1])
+
+
+@pytest.mark.parametrize("nx,ny,tmax", [(1, 1, 2), (1, 5, 3), (5, 1, 3), (4, 6, 4), (7, 5, 6)])
+def test_fdtd2d_bruteforce_bitwise(nx, ny, tmax):
+    rng = np.random.default_rng(nx * 31 + ny)
+    ex, ey, hz = (rng.random((nx, ny)) for _ in range(3))
+    fict = rng.random(tmax)
+    a, b, c = ex.copy(), ey.copy(), hz.copy()
+    S.fdtd_2d_kernel(tmax, a, b, c, fict)
+    ra, rb, rc = scalar.fdtd_2d(tmax, ex, ey, hz, fict)
+    assert np.array_equal(a, ra) and np.array_equal(b, rb) and np.array_equal(c, rc)
+
+
+def test_planted_fdtd2d_mistakes_are_caught(monkeypatch):
+    """A wrong coefficient, a flipped neighbour and a dropped term all move
+    the impulse response."""
+    def make(mistake):
+        def kernel(TMAX, ex, ey, hz, fict):
+            for t in range(TMAX):
+                ey[0, :] = fict[t]
+                up = hz[:-1, :] if mistake != "flip" else hz[1:, :]
+                ey[1:, :] = ey[1:, :] - 0.5 * (hz[1:, :] - up)
+                ex[:, 1:] = ex[:, 1:] - (0.6 if mistake == "coef" else 0.5) * (hz[:, 1:] - hz[:, :-1])
+                curl = ex[:-1, 1:] - ex[:-1, :-1]
+                if mistake != "drop":
+                    curl = curl + ey[1:, :-1]
+                curl = curl - ey[:-1, :-1]
+                hz[:-1, :-1] = hz[:-1, :-1] - 0.7 * curl
+        return kernel
+    for mistake in ("coef", "flip", "drop"):
+        monkeypatch.setattr(S, "fdtd_2d_kernel", make(mistake))
+        with pytest.raises(AssertionError):
+            test_fdtd2d_impulse_response()
+
+
 # ------------------------------------------------------------------ golden
 def _golden(name):
     with np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False) as z:
@@ -533,6 +637,15 @@ def test_golden_jacobi_1d(name):
     a, b = g["A"].copy(), g["B"].copy()
     S.jacobi_1d_kernel(int(g["TSTEPS"]), a, b)
     assert np.array_equal(a, g["A_out"]) and np.array_equal(b, g["B_out"])
+
+
+@pytest.mark.parametrize("name", ["fdtd_2d_ref", "fdtd_2d_uniform"])
+def test_golden_fdtd_2d(name):
+    g = _golden(name)
+    ex, ey, hz = g["ex"].copy(), g["ey"].copy(), g["hz"].copy()
+    S.fdtd_2d_kernel(int(g["TMAX"]), ex, ey, hz, g["fict"])
+    assert np.array_equal(ex, g["ex_out"]) and np.array_equal(ey, g["ey_out"])
+    assert np.array_equal(hz, g["hz_out"])
 
 
 @pytest.mark.parametrize("name", ["adi_ref", "adi_uniform"])
