@@ -380,6 +380,31 @@ def kernel_suite(st, dev, torch, clock, reps, copy_only=False):
             "glups": round(TF * NX * NY * reps / ev / 1e9, 2),
             "field_pass_GBps": round(9 * 8 * TF * NX * NY * reps / ev / 1e9, 1)}
         del f0, f1
+
+    # lid-driven cavity flow (FP64) at the paper and L presets, from the
+    # reference initialisation (fields at rest) on every call: one workgroup
+    # runs nt * (nit + 3) barrier-separated phases; a "lup" is one interior
+    # point of one source, pressure or velocity phase
+    from npbench_amd.benchmarks.cavity_flow import cavity_flow as cf_init
+    for preset, NYC, NXC, NT, NIT in (("paper", 101, 101, 700, 50), ("L", 201, 201, 100, 20)):
+        u, v, p, dx, dy, dt = cf_init.initialize(NYC, NXC)
+        c0 = [dev.to_device(x, torch.float64) for x in (u, v, p)]
+        c1 = [x.clone() for x in c0]
+
+        def cf():
+            for x, y in zip(c1, c0):
+                x.copy_(y)
+            st.cavity_flow(NXC, NYC, NT, NIT, *c1[:2], dt, dx, dy, c1[2], 1.0, 0.1,
+                           stream=clock.stream)
+        cf()
+        _, ev = clock.run(cf, reps)
+        phases = NT * (NIT + 3)
+        res[f"cavity_flow_{preset}_{NYC}x{NXC}_nt{NT}_nit{NIT}_f64"] = {
+            "ms_per_call": round(ev / reps * 1e3, 4),
+            "us_per_phase": round(ev / reps / phases * 1e6, 3),
+            "glups": round(NT * (NIT + 2) * (NXC - 2) * (NYC - 2) * reps / ev / 1e9, 3),
+            "bound": "latency (one workgroup, a barrier per phase)"}
+        del c0, c1
     return res
 
 
@@ -425,6 +450,18 @@ def cpu_fdtd2d_ms():
         ref.fdtd_2d_kernel(150, ex, ey, hz, fict)
         times.append(time.perf_counter() - t0)
     return round(statistics.median(times) * 1e3, 1)
+
+
+def cpu_cavity_ms(sample_nt=70):
+    """The NumPy oracle's cavity_flow at the paper preset (101 x 101, nit=50;
+    1 core) on a bounded sample of sample_nt of its 700 time steps, scaled to
+    700: every step costs the same (fixed nit, no data-dependent work)."""
+    from oracle import stencils as ref
+    from npbench_amd.benchmarks.cavity_flow import cavity_flow as cf_init
+    u, v, p, dx, dy, dt = cf_init.initialize(101, 101)
+    t0 = time.perf_counter()
+    ref.cavity_flow(101, 101, sample_nt, 50, u, v, dt, dx, dy, p, 1.0, 0.1)
+    return round((time.perf_counter() - t0) * 1e3 * 700 / sample_nt, 1)
 
 
 def cpu_adi_ms():
@@ -565,6 +602,10 @@ def main():
             fd = line["kernels"]["fdtd_2d_L_800x900_T150_f64"]
             fd["cpu_oracle_ms"] = cpu_fdtd2d_ms()
             fd["speedup_vs_cpu_oracle"] = round(fd["cpu_oracle_ms"] / fd["ms_per_call"], 1)
+            cf = line["kernels"]["cavity_flow_paper_101x101_nt700_nit50_f64"]
+            cf["cpu_oracle_ms"] = cpu_cavity_ms()
+            cf["cpu_oracle_sample"] = "nt=70 of 700 steps, scaled by 10"
+            cf["speedup_vs_cpu_oracle"] = round(cf["cpu_oracle_ms"] / cf["ms_per_call"], 1)
     if args.cpu_sweeps > 0:
         line["cpu_baseline"] = cpu_baseline(HEAT_N, args.cpu_sweeps)
     print(json.dumps(line), flush=True)

@@ -235,6 +235,17 @@ int npbx_jacobi1d(int dtype, void *A, void *B, int64_t N, int64_t tsteps,
 int npbx_fdtd2d(int dtype, void *ex, void *ey, void *hz, const void *fict,
                 int64_t NX, int64_t NY, int64_t tmax, void *stream);
 
+/* ---- lid-driven cavity flow (cavity_flow_numpy.py:13-83) -------------------
+ * u, v, p: (ny, nx) FP64 (dtype NPBX_F64), updated in place by nt time steps
+ * of nit pressure iterations each, with the reference's argument order and
+ * scalars (cavity_flow_numpy.py:41); bitwise the NumPy result.  3 <= nx, ny
+ * and nx * ny <= 2^26; nt == 0 is a no-op.  One workgroup runs the whole
+ * call; grids of up to 10240 points keep the pressure in LDS.  A device
+ * workspace of 4 nx ny doubles is reused between calls. */
+int npbx_cavity_flow(int dtype, int64_t nx, int64_t ny, int64_t nt, int64_t nit,
+                     void *u, void *v, double dt, double dx, double dy, void *p,
+                     double rho, double nu, void *stream);
+
 /* ---- alternating-direction implicit solver (adi_numpy.py:6-52) -------------
  * u: (N, N) FP64 (dtype NPBX_F64), updated in place by TSTEPS time steps of
  * the reference, each a column sweep into library scratch and a row sweep

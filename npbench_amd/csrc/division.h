@@ -1,0 +1,90 @@
+// FP64 division by a denominator whose reciprocal is known ahead, with the
+// exact bits of the compiler's IEEE division sequence (v_div_scale, v_rcp,
+// two Newton steps, v_div_fmas, v_div_fixup) whenever that sequence does not
+// rescale; callers check the bounds below and fall back to '/' otherwise.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace npbx {
+
+// The reciprocal that the compiler's IEEE FP64 division sequence computes
+// from an unscaled denominator: the hardware estimate refined by two Newton
+// steps.  With it the sequence ends m = n y, r = fma(-den, m, n),
+// q = fma(r, y, m) (its div_fmas and div_fixup steps return their operand
+// when nothing was rescaled and the result is a normal number).
+__device__ __forceinline__ double division_reciprocal(double den) {
+  double y = __builtin_amdgcn_rcp(den);
+  double e = __builtin_fma(-den, y, 1.0);
+  y = __builtin_fma(y, e, y);
+  e = __builtin_fma(-den, y, 1.0);
+  return __builtin_fma(y, e, y);
+}
+
+__device__ __forceinline__ double short_division(double n, double den, double y, double& m) {
+  m = n * y;
+  const double r = __builtin_fma(-den, m, n);
+  return __builtin_fma(r, y, m);
+}
+
+__device__ __forceinline__ int exponent_of(double x) {
+  return (__double2hiint(x) >> 20) & 0x7ff;
+}
+
+// den's side of the no-rescale condition: a normal number whose reciprocal
+// is normal, with room for the numerator bounds below.
+__device__ __forceinline__ bool den_in_range(double den) {
+  const int e = exponent_of(den);
+  return e >= 100 && e <= 1900;
+}
+
+// The numerator's side, for dens in range of magnitude dmin or more: the
+// division sequence rescales when n is zero, tiny (biased exponent <= 53),
+// not finite, at least 2^768 times den (in exponent), or when the quotient
+// is subnormal.  With m = n y, the quotient to within a few units of its
+// last place, lo <= |m| < hi excludes all of them:
+//   hi = 2^765: |n / den| < 2^766, so the exponents differ by at most 766;
+//   lo = max(2^-1020, 2^-960 / dmin): the quotient is normal, and
+//        |n| = |m den| >= 2^-960 (biased exponent above 53).
+// A NaN fails both comparisons, an infinite n gives an infinite m.
+__device__ __forceinline__ void quotient_bounds(double dmin, double& lo, double& hi) {
+  lo = fmax(0x1p-1020, 0x1p-960 / dmin);
+  hi = 0x1p765;
+}
+
+__device__ __forceinline__ bool unscaled_division(double m, double lo, double hi) {
+  const double a = fabs(m);
+  return a >= lo && a < hi;
+}
+
+// A uniform divisor prepared once: its reciprocal, the quotient bounds of the
+// no-rescale condition (an empty range when the short form never applies),
+// and its sign.
+struct Divisor {
+  double d, y, lo, hi, sgn;
+  bool fast;
+};
+
+__device__ __forceinline__ Divisor make_divisor(double d) {
+  Divisor v;
+  v.d = d;
+  v.fast = den_in_range(d);
+  v.y = v.fast ? division_reciprocal(d) : 0.0;
+  quotient_bounds(fabs(d), v.lo, v.hi);
+  if (!v.fast) v.lo = INFINITY;
+  v.sgn = d > 0.0 ? 1.0 : -1.0;
+  return v;
+}
+
+// n / v.d, bit for bit, with the full sequence wherever the short form
+// cannot vouch for the result; a zero numerator over an in-range divisor is
+// the quotient's zero (n times the divisor's sign).
+__device__ __forceinline__ double divide(double n, const Divisor& v) {
+  double m;
+  const double q = short_division(n, v.d, v.y, m);
+  if (unscaled_division(m, v.lo, v.hi)) return q;
+  if (v.fast && n == 0.0) return n * v.sgn;
+  return n / v.d;
+}
+
+}  // namespace npbx

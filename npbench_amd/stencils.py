@@ -3,10 +3,12 @@
 Two layers:
 
 * resident operations (``jacobi2d_sweeps``, ``heat3d_sweeps``, ``hdiff``,
-  ``vadv``, ``adi``, ``jacobi1d``, ``fdtd2d``) take device tensors already in their storage dtype (float64,
+  ``vadv``, ``adi``, ``jacobi1d``, ``fdtd2d``, ``cavity_flow``) take device
+  tensors already in their storage dtype (float64,
   float32 or bfloat16) and enqueue HIP kernels on a stream without waiting;
 * drop-in operations (``jacobi_2d``, ``heat_3d``, ``hdiff_dropin``,
-  ``vadv_dropin``, ``adi_dropin``, ``jacobi_1d``, ``fdtd_2d``) take the reference's float64 arrays -- device tensors
+  ``vadv_dropin``, ``adi_dropin``, ``jacobi_1d``, ``fdtd_2d``,
+  ``cavity_flow_dropin``) take the reference's float64 arrays -- device tensors
   produced by the framework's copy function, or host NumPy arrays, which are
   copied to the device, processed and copied back in place -- and a
   ``precision`` that picks the storage form.  FP32/BF16 forms convert to the
@@ -109,6 +111,16 @@ def fdtd2d(ex, ey, hz, fict, tmax, stream=None):
         raise ValueError(f"fdtd2d: fict needs at least TMAX={int(tmax)} entries")
     nat.call("npbx_fdtd2d", dev.dtype_code(ex), dev.ptr(ex), dev.ptr(ey), dev.ptr(hz),
              dev.ptr(fict), ex.shape[0], ex.shape[1], int(tmax), dev.stream_handle(stream))
+
+
+def cavity_flow(nx, ny, nt, nit, u, v, dt, dx, dy, p, rho, nu, stream=None):
+    """cavity_flow_numpy.cavity_flow on float64 (ny, nx) device tensors u, v,
+    p (in place); the reference's argument order."""
+    for name, t in (("u", u), ("v", v), ("p", p)):
+        dev.check_device_array(name, t, shape=(int(ny), int(nx)), dtype=torch.float64)
+    nat.call("npbx_cavity_flow", dev.dtype_code(u), int(nx), int(ny), int(nt), int(nit),
+             dev.ptr(u), dev.ptr(v), float(dt), float(dx), float(dy), dev.ptr(p), float(rho),
+             float(nu), dev.stream_handle(stream))
 
 
 def cast_into(src, dst, stream=None):
@@ -222,3 +234,9 @@ def fdtd_2d(TMAX, ex, ey, hz, _fict_):
     place."""
     with _Staged("fp64", [ex, ey, hz, _fict_], outputs=(0, 1, 2)) as (a, b, c, f):
         fdtd2d(a, b, c, f, TMAX)
+
+
+def cavity_flow_dropin(nx, ny, nt, nit, u, v, dt, dx, dy, p, rho, nu):
+    """cavity_flow_numpy.cavity_flow(...): u, v and p updated in place."""
+    with _Staged("fp64", [u, v, p], outputs=(0, 1, 2)) as (a, b, c):
+        cavity_flow(nx, ny, nt, nit, a, b, dt, dx, dy, c, rho, nu)

@@ -9,7 +9,8 @@ returns new arrays and leaves its inputs alone.
 
 References: jacobi_2d_numpy.py:4-10, heat_3d_numpy.py:4-20,
 hdiff_numpy.py:5-29, vadv_numpy.py:4-78, adi_numpy.py:6-52,
-jacobi_1d_numpy.py:4-8, fdtd_2d_numpy.py:4-11 (npbench/benchmarks/...).
+jacobi_1d_numpy.py:4-8, fdtd_2d_numpy.py:4-11, cavity_flow_numpy.py:13-83
+(npbench/benchmarks/...).
 """
 
 import numpy as np
@@ -248,3 +249,73 @@ def fdtd_2d(TMAX, ex, ey, hz, fict):
             for j in range(ny - 1):
                 hz[i][j] = hz[i][j] - 0.7 * (ex[i][j + 1] - ex[i][j] + ey[i + 1][j] - ey[i][j])
     return np.array(ex), np.array(ey), np.array(hz)
+
+
+def cavity_flow(nx, ny, nt, nit, u, v, dt, dx, dy, p, rho, nu):
+    """cavity_flow_numpy.py:13-83, one point at a time; every statement of
+    the reference, boundary copies included, runs in its own order.  Returns
+    new (u, v, p)."""
+    U = u.tolist()
+    V = v.tolist()
+    P = p.tolist()
+    B = [[0.0] * nx for _ in range(ny)]
+    for _ in range(nt):
+        un = [r[:] for r in U]
+        vn = [r[:] for r in V]
+        for i in range(1, ny - 1):  # build_up_b
+            for j in range(1, nx - 1):
+                dudx = (U[i][j + 1] - U[i][j - 1]) / (2 * dx)
+                dvdy = (V[i + 1][j] - V[i - 1][j]) / (2 * dy)
+                dudy = (U[i + 1][j] - U[i - 1][j]) / (2 * dy)
+                cross = dudy * (V[i][j + 1] - V[i][j - 1]) / (2 * dx)
+                t = (1 / dt) * (dudx + dvdy)
+                t = t - dudx * dudx
+                t = t - 2 * cross
+                t = t - dvdy * dvdy
+                B[i][j] = rho * t
+        for _q in range(nit):  # pressure_poisson
+            pn = [r[:] for r in P]
+            d2x = dx ** 2
+            d2y = dy ** 2
+            den = 2 * (d2x + d2y)
+            for i in range(1, ny - 1):
+                for j in range(1, nx - 1):
+                    num = (pn[i][j + 1] + pn[i][j - 1]) * d2y + (pn[i + 1][j] + pn[i - 1][j]) * d2x
+                    P[i][j] = num / den - (d2x * d2y / den) * B[i][j]
+            for i in range(ny):
+                P[i][nx - 1] = P[i][nx - 2]
+            for j in range(nx):
+                P[0][j] = P[1][j]
+            for i in range(ny):
+                P[i][0] = P[i][1]
+            for j in range(nx):
+                P[ny - 1][j] = 0.0
+        for i in range(1, ny - 1):
+            for j in range(1, nx - 1):
+                uc, vc = un[i][j], vn[i][j]
+                x = uc - ((uc * dt) / dx) * (uc - un[i][j - 1])
+                x = x - ((vc * dt) / dy) * (uc - un[i - 1][j])
+                x = x - (dt / (2 * rho * dx)) * (P[i][j + 1] - P[i][j - 1])
+                lap = (dt / dx ** 2) * ((un[i][j + 1] - 2 * uc) + un[i][j - 1])
+                lap = lap + (dt / dy ** 2) * ((un[i + 1][j] - 2 * uc) + un[i - 1][j])
+                U[i][j] = x + nu * lap
+                y = vc - ((uc * dt) / dx) * (vc - vn[i][j - 1])
+                y = y - ((vc * dt) / dy) * (vc - vn[i - 1][j])
+                y = y - (dt / (2 * rho * dy)) * (P[i + 1][j] - P[i - 1][j])
+                lap = (dt / dx ** 2) * ((vn[i][j + 1] - 2 * vc) + vn[i][j - 1])
+                lap = lap + (dt / dy ** 2) * ((vn[i + 1][j] - 2 * vc) + vn[i - 1][j])
+                V[i][j] = y + nu * lap
+        for j in range(nx):
+            U[0][j] = 0.0
+        for i in range(ny):
+            U[i][0] = 0.0
+            U[i][nx - 1] = 0.0
+        for j in range(nx):
+            U[ny - 1][j] = 1.0
+        for j in range(nx):
+            V[0][j] = 0.0
+            V[ny - 1][j] = 0.0
+        for i in range(ny):
+            V[i][0] = 0.0
+            V[i][nx - 1] = 0.0
+    return np.array(U), np.array(V), np.array(P)

@@ -12,6 +12,7 @@ Reference files (paths relative to the npbench source tree):
   adi_kernel        npbench/benchmarks/polybench/adi/adi_numpy.py:6-52
   jacobi_1d_kernel  npbench/benchmarks/polybench/jacobi_1d/jacobi_1d_numpy.py:4-8
   fdtd_2d_kernel    npbench/benchmarks/polybench/fdtd_2d/fdtd_2d_numpy.py:4-11
+  cavity_flow       npbench/benchmarks/cavity_flow/cavity_flow_numpy.py:13-83
 """
 
 import numpy as np
@@ -355,3 +356,58 @@ def fdtd_2d_kernel(TMAX, ex, ey, hz, _fict_):
         curl = curl + ey[1:, :-1]
         curl = curl - ey[:-1, :-1]
         hz[:-1, :-1] = hz[:-1, :-1] - 0.7 * curl
+
+
+# --------------------------------------------------------------------------
+# Lid-driven cavity flow
+# --------------------------------------------------------------------------
+def _cavity_b(b, rho, dt, u, v, dx, dy):
+    """build_up_b (cavity_flow_numpy.py:13-20): the pressure equation's source
+    from the velocity field's central differences, interior only."""
+    dudx = (u[1:-1, 2:] - u[1:-1, :-2]) / (2 * dx)
+    dvdy = (v[2:, 1:-1] - v[:-2, 1:-1]) / (2 * dy)
+    cross = (u[2:, 1:-1] - u[:-2, 1:-1]) / (2 * dy) * (v[1:-1, 2:] - v[1:-1, :-2]) / (2 * dx)
+    b[1:-1, 1:-1] = rho * (1 / dt * (dudx + dvdy) - dudx ** 2 - 2 * cross - dvdy ** 2)
+
+
+def _cavity_pressure(nit, p, dx, dy, b):
+    """pressure_poisson (cavity_flow_numpy.py:23-33): nit Jacobi iterations,
+    each followed by the four boundary statements in the reference's order."""
+    den = 2 * (dx ** 2 + dy ** 2)
+    for _ in range(nit):
+        pn = p.copy()
+        avg = ((pn[1:-1, 2:] + pn[1:-1, :-2]) * dy ** 2 + (pn[2:, 1:-1] + pn[:-2, 1:-1]) * dx ** 2) / den
+        p[1:-1, 1:-1] = avg - dx ** 2 * dy ** 2 / den * b[1:-1, 1:-1]
+        p[:, -1] = p[:, -2]
+        p[0, :] = p[1, :]
+        p[:, 0] = p[:, 1]
+        p[-1, :] = 0
+
+
+def cavity_flow(nx, ny, nt, nit, u, v, dt, dx, dy, p, rho, nu):
+    """cavity_flow_numpy.cavity_flow: nt time steps, u, v, p in place."""
+    _f64(u, v, p)
+    b = np.zeros((ny, nx))
+    for _ in range(nt):
+        un = u.copy()
+        vn = v.copy()
+        _cavity_b(b, rho, dt, u, v, dx, dy)
+        _cavity_pressure(nit, p, dx, dy, b)
+        c, w, e, s_, n_ = (slice(1, -1), slice(0, -2), slice(2, None), slice(0, -2), slice(2, None))
+        uc, vc = un[c, c], vn[c, c]
+        u[c, c] = (uc - uc * dt / dx * (uc - un[c, w]) - vc * dt / dy * (uc - un[s_, c])
+                   - dt / (2 * rho * dx) * (p[c, e] - p[c, w])
+                   + nu * (dt / dx ** 2 * (un[c, e] - 2 * uc + un[c, w])
+                           + dt / dy ** 2 * (un[n_, c] - 2 * uc + un[s_, c])))
+        v[c, c] = (vc - uc * dt / dx * (vc - vn[c, w]) - vc * dt / dy * (vc - vn[s_, c])
+                   - dt / (2 * rho * dy) * (p[n_, c] - p[s_, c])
+                   + nu * (dt / dx ** 2 * (vn[c, e] - 2 * vc + vn[c, w])
+                           + dt / dy ** 2 * (vn[n_, c] - 2 * vc + vn[s_, c])))
+        u[0, :] = 0
+        u[:, 0] = 0
+        u[:, -1] = 0
+        u[-1, :] = 1
+        v[0, :] = 0
+        v[-1, :] = 0
+        v[:, 0] = 0
+        v[:, -1] = 0

@@ -21,6 +21,10 @@ Cases (all FP64):
               seeded uniform
   jacobi_1d   N=10 TSTEPS=4 reference init; N=9 TSTEPS=5 seeded uniform
   fdtd_2d     (NX, NY) = (5, 6) TMAX=3 reference init; (4, 7) TMAX=4 seeded
+  cavity_flow (ny, nx) = (6, 8) nt=4 nit=3 reference init (zeros, the lid
+              drives the flow); (5, 7) nt=3 nit=4 seeded u, v, p
+
+Arguments, if given, are name prefixes: only the matching files are written.
 """
 
 import os
@@ -40,9 +44,14 @@ from npbench_amd.benchmarks.weather_stencils.vadv import vadv  # noqa: E402
 from npbench_amd.benchmarks.polybench.adi import adi  # noqa: E402
 from npbench_amd.benchmarks.polybench.jacobi_1d import jacobi_1d  # noqa: E402
 from npbench_amd.benchmarks.polybench.fdtd_2d import fdtd_2d  # noqa: E402
+from npbench_amd.benchmarks.cavity_flow import cavity_flow  # noqa: E402
+
+ONLY = tuple(sys.argv[1:])
 
 
 def save(name, **arrays):
+    if ONLY and not name.startswith(ONLY):
+        return
     path = os.path.join(HERE, name + ".npz")
     np.savez_compressed(path, **arrays)
     print(f"wrote {os.path.relpath(path, ROOT)} ({os.path.getsize(path)} bytes)")
@@ -107,6 +116,17 @@ def main():
         oex, oey, ohz = scalar.fdtd_2d(tmax, ex, ey, hz, fict)
         save(f"fdtd_2d_{tag}", TMAX=tmax, ex=ex, ey=ey, hz=hz, fict=fict, ex_out=oex,
              ey_out=oey, hz_out=ohz)
+
+    # cavity flow
+    rng = np.random.default_rng(19)
+    for tag, (ny, nx, nt, nit), init in (("ref", (6, 8, 4, 3), None),
+                                         ("uniform", (5, 7, 3, 4), 1)):
+        u, v, p, dx, dy, dt = cavity_flow.initialize(ny, nx)
+        if init:
+            u, v, p = rng.random((ny, nx)), rng.random((ny, nx)) - 0.5, rng.random((ny, nx))
+        ou, ov, op = scalar.cavity_flow(nx, ny, nt, nit, u, v, dt, dx, dy, p, 1.0, 0.1)
+        save(f"cavity_flow_{tag}", nx=nx, ny=ny, nt=nt, nit=nit, dt=dt, dx=dx, dy=dy, rho=1.0,
+             nu=0.1, u=u, v=v, p=p, u_out=ou, v_out=ov, p_out=op)
 
 
 if __name__ == "__main__":

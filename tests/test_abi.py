@@ -62,6 +62,14 @@ def test_version_string():
     (lambda: nat.call("npbx_jacobi1d", 0, None, None, 10, 5, None), "distinct"),
     (lambda: nat.call("npbx_fdtd2d", 1, None, None, None, None, 4, 4, 2, None), "dtype"),
     (lambda: nat.call("npbx_fdtd2d", 0, None, None, None, None, 4, 4, 2, None), "null"),
+    (lambda: nat.call("npbx_cavity_flow", 1, 5, 5, 2, 2, None, None, 0.1, 0.5, 0.5, None, 1.0,
+                      0.1, None), "dtype"),
+    (lambda: nat.call("npbx_cavity_flow", 0, 2, 5, 2, 2, None, None, 0.1, 0.5, 0.5, None, 1.0,
+                      0.1, None), "out of range"),
+    (lambda: nat.call("npbx_cavity_flow", 0, 5, 5, -1, 2, None, None, 0.1, 0.5, 0.5, None, 1.0,
+                      0.1, None), "negative"),
+    (lambda: nat.call("npbx_cavity_flow", 0, 5, 5, 2, 2, None, None, 0.1, 0.5, 0.5, None, 1.0,
+                      0.1, None), "distinct"),
 ])
 def test_argument_errors_raise_with_message(call, needle):
     with pytest.raises(nat.NpbxError) as ei:
@@ -81,6 +89,8 @@ def test_empty_work_is_a_no_op():
     assert nat.call("npbx_jacobi1d", 0, None, None, 10, 1, None) == nat.OK  # no sweep
     assert nat.call("npbx_fdtd2d", 0, None, None, None, None, 0, 5, 3, None) == nat.OK
     assert nat.call("npbx_fdtd2d", 0, None, None, None, None, 4, 5, 0, None) == nat.OK
+    assert nat.call("npbx_cavity_flow", 0, 5, 5, 0, 3, None, None, 0.1, 0.5, 0.5, None, 1.0, 0.1,
+                    None) == nat.OK  # no time step
 
 
 def test_last_error_buffer_is_terminated():

@@ -34,6 +34,15 @@ fails at least one check:
              fields are a fixed point; ey's first row holds the last source
              value; linearity; fields that vary along i only keep ex at zero
              and every column equal; brute force and golden vectors.
+  cavity     from rest, the lid's first viscous step puts nu*dt/dy^2 on the
+             row below it and nothing else (dx != dy); a pressure ramp along
+             x or y drives only u or only v by dt/(2 rho d)*2; pressure
+             iterations from p = 1 relax the row under the p = 0 lid to
+             (2dy^2 + dx^2)/(2(dx^2 + dy^2)) on the second iteration, not the
+             first; velocity fields with exact difference quotients give the
+             source term's closed forms (the divergence and the cross term)
+             through p = -dx^2 dy^2/(2(dx^2 + dy^2)) b; linear fields advect
+             by u du/dx and v dv/dy; brute force and golden vectors.
 
 The last section plants mistakes in the oracle (monkeypatched copies) and
 checks the pins above catch them.
@@ -596,6 +605,160 @@ def test_planted_fdtd2d_mistakes_are_caught(monkeypatch):
             test_fdtd2d_impulse_response()
 
 
+# ------------------------------------------------------------------ cavity flow
+def cavity_run(ny, nx, nt, nit, u=None, v=None, p=None, rho=1.0, nu=0.1):
+    """Runs S.cavity_flow (looked up per call: planted mistakes patch it) on
+    the reference grid spacing and returns (u, v, p, dx, dy, dt)."""
+    dx, dy, dt = 2 / (nx - 1), 2 / (ny - 1), .1 / ((nx - 1) * (ny - 1))
+    u = np.zeros((ny, nx)) if u is None else u.copy()
+    v = np.zeros((ny, nx)) if v is None else v.copy()
+    p = np.zeros((ny, nx)) if p is None else p.copy()
+    S.cavity_flow(nx, ny, nt, nit, u, v, dt, dx, dy, p, rho, nu)
+    return u, v, p, dx, dy, dt
+
+
+def test_cavity_lid_diffusion():
+    """From rest, step 1 only sets the lid; in step 2 nothing but the lid's
+    viscous term moves the row below it: u = nu dt / dy^2 there (dx != dy),
+    and v and p stay zero."""
+    ny, nx, nu = 6, 9, 0.1
+    u, v, p, dx, dy, dt = cavity_run(ny, nx, 1, 3, nu=nu)
+    assert np.all(u[-1] == 1.0) and np.all(u[:-1] == 0.0) and not v.any() and not p.any()
+    u, v, p, dx, dy, dt = cavity_run(ny, nx, 2, 3, nu=nu)
+    assert dx != dy
+    assert np.all(u[-2, 1:-1] == nu * (dt / dy ** 2))
+    assert np.all(u[1:-2] == 0.0) and np.all(u[:, [0, -1]][:-1] == 0.0) and np.all(u[-1] == 1.0)
+    assert not v.any() and not p.any()
+
+
+def test_cavity_pressure_gradient():
+    """With no flow and no pressure iterations, a pressure ramp along x
+    (columns) moves only u and one along y (rows) only v, each by
+    -dt/(2 rho d) (p[+1] - p[-1])."""
+    ny, nx, rho = 9, 5, 2.0
+    i, j = np.indices((ny, nx), dtype=np.float64)
+    u, v, p, dx, dy, dt = cavity_run(ny, nx, 1, 0, p=3 * j, rho=rho, nu=0.0)
+    assert np.allclose(u[1:-1, 1:-1], -dt / (2 * rho * dx) * 6, rtol=1e-15, atol=0)
+    assert not v.any() and np.array_equal(p, 3 * j)
+    u, v, p, dx, dy, dt = cavity_run(ny, nx, 1, 0, p=3 * i, rho=rho, nu=0.0)
+    assert np.allclose(v[1:-1, 1:-1], -dt / (2 * rho * dy) * 6, rtol=1e-15, atol=0)
+    assert not u[:-1].any()
+
+
+def test_cavity_pressure_iterations():
+    """No flow (b = 0), p = 1 everywhere: the first iteration keeps p = 1 but
+    zeroes the lid row; the second relaxes the row below to
+    (2 dy^2 + dx^2) / (2 (dx^2 + dy^2)) and leaves the rest at 1, boundary
+    copies included (corner (0, 0) and column nx-1 too)."""
+    ny, nx = 7, 11
+    one = np.ones((ny, nx))
+    u, v, p, dx, dy, dt = cavity_run(ny, nx, 1, 1, p=one)
+    assert np.allclose(p[:-1], 1.0, rtol=0, atol=2e-16) and np.all(p[-1] == 0.0)
+    u, v, p, dx, dy, dt = cavity_run(ny, nx, 1, 2, p=one)
+    relaxed = (2 * dy ** 2 + dx ** 2) / (2 * (dx ** 2 + dy ** 2))
+    assert abs(relaxed - 1.0) > 0.05
+    assert np.allclose(p[-2], relaxed, rtol=1e-15, atol=0)
+    assert np.allclose(p[:-2], 1.0, rtol=0, atol=4e-16) and np.all(p[-1] == 0.0)
+
+
+def test_cavity_source_term():
+    """Velocity fields whose difference quotients are exact (dx = 1/2,
+    dy = 1/4): p after one iteration from 0 is -c b with
+    c = dx^2 dy^2 / (2 (dx^2 + dy^2)), where b = rho (1/dt (ux + vy) - ux^2 -
+    2 uy vx - vy^2)."""
+    ny, nx, rho = 9, 5, 1.5
+    i, j = np.indices((ny, nx), dtype=np.float64)
+    dx, dy = 0.5, 0.25
+    c = dx ** 2 * dy ** 2 / (2 * (dx ** 2 + dy ** 2))
+    # the cross term: u varies along y only, v along x only
+    u, v, p, dx_, dy_, dt = cavity_run(ny, nx, 1, 1, u=3 * i, v=5 * j, rho=rho)
+    assert (dx_, dy_) == (dx, dy)
+    b = rho * (-2 * (6 / (2 * dy)) * (10 / (2 * dx)))
+    assert np.allclose(p[1:-1, 1:-1], -c * b, rtol=1e-14, atol=0)
+    # the divergence and its squares: u along x, v along y
+    u, v, p, _, _, dt = cavity_run(ny, nx, 1, 1, u=3 * j, v=5 * i, rho=rho)
+    ux, vy = 6 / (2 * dx), 10 / (2 * dy)
+    b = rho * ((ux + vy) / dt - ux ** 2 - vy ** 2)
+    assert np.allclose(p[1:-1, 1:-1], -c * b, rtol=1e-14, atol=0)
+
+
+def test_cavity_advection():
+    """Linear fields u = a x, v = b y with no pressure and no viscosity: one
+    step gives u (1 - a dt/dx') and v (1 - b dt/dy') in grid units (the upwind
+    difference of a linear field is exact), and the boundary conditions."""
+    ny, nx = 9, 5
+    i, j = np.indices((ny, nx), dtype=np.float64)
+    u, v, p, dx, dy, dt = cavity_run(ny, nx, 1, 0, u=0.5 * j, v=0.25 * i, nu=0.0)
+    assert np.allclose(u[1:-1, 1:-1], (0.5 * j * (1 - 0.5 * dt / dx))[1:-1, 1:-1], rtol=1e-15, atol=0)
+    assert np.allclose(v[1:-1, 1:-1], (0.25 * i * (1 - 0.25 * dt / dy))[1:-1, 1:-1], rtol=1e-15,
+                       atol=0)
+    assert np.all(u[-1] == 1.0) and not u[:-1, [0, -1]].any() and not u[0].any()
+    assert not v[[0, -1]].any() and not v[:, [0, -1]].any()
+
+
+@pytest.mark.parametrize("ny,nx,nt,nit", [(3, 3, 2, 1), (5, 7, 3, 4), (6, 4, 2, 3), (8, 9, 3, 0),
+                                          (7, 5, 3, 5)])
+def test_cavity_bruteforce_bitwise(ny, nx, nt, nit):
+    rng = np.random.default_rng(ny * 37 + nx)
+    u, v, p = rng.random((ny, nx)), rng.random((ny, nx)) - 0.5, rng.random((ny, nx))
+    a, b, c, dx, dy, dt = cavity_run(ny, nx, nt, nit, u, v, p, rho=1.25, nu=0.07)
+    ra, rb, rc = scalar.cavity_flow(nx, ny, nt, nit, u, v, dt, dx, dy, p, 1.25, 0.07)
+    assert np.array_equal(a, ra) and np.array_equal(b, rb) and np.array_equal(c, rc)
+
+
+CAVITY_PINS = (test_cavity_lid_diffusion, test_cavity_pressure_gradient,
+               test_cavity_pressure_iterations, test_cavity_source_term, test_cavity_advection)
+
+
+def _caught(pins):
+    for pin in pins:
+        try:
+            pin()
+        except AssertionError:
+            return True
+    return False
+
+
+def test_planted_cavity_mistakes_are_caught(monkeypatch):
+    """A halved cross term, swapped Poisson weights, a pn copied once instead
+    of per iteration, and a flipped advection neighbour each fail a pin."""
+    good_b, good_p, good_flow = S._cavity_b, S._cavity_pressure, S.cavity_flow
+
+    def half_cross(b, rho, dt, u, v, dx, dy):
+        good_b(b, rho, dt, u, v, dx, dy)
+        cross = (u[2:, 1:-1] - u[:-2, 1:-1]) / (2 * dy) * (v[1:-1, 2:] - v[1:-1, :-2]) / (2 * dx)
+        b[1:-1, 1:-1] += rho * cross
+
+    def swapped_weights(nit, p, dx, dy, b):
+        good_p(nit, p, dy, dx, b)
+
+    def stale_pn(nit, p, dx, dy, b):
+        pn = p.copy()
+        den = 2 * (dx ** 2 + dy ** 2)
+        for _ in range(nit):
+            p[1:-1, 1:-1] = (((pn[1:-1, 2:] + pn[1:-1, :-2]) * dy ** 2
+                              + (pn[2:, 1:-1] + pn[:-2, 1:-1]) * dx ** 2) / den
+                             - dx ** 2 * dy ** 2 / den * b[1:-1, 1:-1])
+            p[:, -1] = p[:, -2]
+            p[0, :] = p[1, :]
+            p[:, 0] = p[:, 1]
+            p[-1, :] = 0
+
+    def flipped_upwind(nx, ny, nt, nit, u, v, dt, dx, dy, p, rho, nu):
+        # mirror the field along x, step it, mirror back: the upwind
+        # neighbour becomes the east one
+        uf, vf, pf = u[:, ::-1].copy(), v[:, ::-1].copy(), p[:, ::-1].copy()
+        good_flow(nx, ny, nt, nit, uf, vf, dt, dx, dy, pf, rho, nu)
+        u[...], v[...], p[...] = uf[:, ::-1], vf[:, ::-1], pf[:, ::-1]
+
+    for name, bad in (("_cavity_b", half_cross), ("_cavity_pressure", swapped_weights),
+                      ("_cavity_pressure", stale_pn), ("cavity_flow", flipped_upwind)):
+        with monkeypatch.context() as m:
+            m.setattr(S, name, bad)
+            assert _caught(CAVITY_PINS), name
+    assert all(pin() is None for pin in CAVITY_PINS)
+
+
 # ------------------------------------------------------------------ golden
 def _golden(name):
     with np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False) as z:
@@ -646,6 +809,16 @@ def test_golden_fdtd_2d(name):
     S.fdtd_2d_kernel(int(g["TMAX"]), ex, ey, hz, g["fict"])
     assert np.array_equal(ex, g["ex_out"]) and np.array_equal(ey, g["ey_out"])
     assert np.array_equal(hz, g["hz_out"])
+
+
+@pytest.mark.parametrize("name", ["cavity_flow_ref", "cavity_flow_uniform"])
+def test_golden_cavity_flow(name):
+    g = _golden(name)
+    u, v, p = g["u"].copy(), g["v"].copy(), g["p"].copy()
+    S.cavity_flow(int(g["nx"]), int(g["ny"]), int(g["nt"]), int(g["nit"]), u, v, float(g["dt"]),
+                  float(g["dx"]), float(g["dy"]), p, float(g["rho"]), float(g["nu"]))
+    assert np.array_equal(u, g["u_out"]) and np.array_equal(v, g["v_out"])
+    assert np.array_equal(p, g["p_out"])
 
 
 @pytest.mark.parametrize("name", ["adi_ref", "adi_uniform"])
