@@ -61,26 +61,32 @@ struct CavityArgs {
 };
 
 // The kernel's divisors, prepared once per launch and held in scalar
-// registers: every division of the time step is by one of them.
+// registers: every division of the time step is by one of them.  Only their
+// reciprocals and one common no-rescale range of |m| stay live (an empty
+// range when a divisor is outside den_in_range), so the kernel's uniform
+// values fit the scalar register file.
 __device__ __forceinline__ double uniform(double x) {
   return __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(x)),
                           __builtin_amdgcn_readfirstlane(__double2loint(x)));
 }
 
-__device__ __forceinline__ Divisor uniform_divisor(double d) {
-  Divisor v = make_divisor(d);
-  v.y = uniform(v.y);
-  v.lo = uniform(v.lo);
-  v.hi = uniform(v.hi);
-  v.sgn = uniform(v.sgn);
-  v.fast = __builtin_amdgcn_readfirstlane(v.fast ? 1 : 0) != 0;
-  return v;
-}
+struct Divisor {
+  double d, y;
+};
 
 struct CavityDivisors {
   Divisor den, two_dx, two_dy, dx, dy;
-  double lo, hi;  // the no-rescale range of |m| common to all five
+  double lo, hi;  // |m| in [lo, hi) for every division of a chunk: exact
 };
+
+__device__ __forceinline__ Divisor prepare_divisor(double d, double& lo, double& hi) {
+  double dlo, dhi;
+  quotient_bounds(fabs(d), dlo, dhi);
+  const bool fast = den_in_range(d);
+  lo = fmax(lo, fast ? dlo : INFINITY);
+  hi = fmin(hi, dhi);
+  return {d, uniform(fast ? division_reciprocal(d) : 0.0)};
+}
 
 // Walks this thread's interior points k = t, t + kThreadsC, ... in row-major
 // order of the (ny-2) x (nx-2) interior without a division per point, C
@@ -131,8 +137,8 @@ constexpr int kMaxPts = kLdsPoints / kThreadsC;  // interior points per thread, 
 // The point computations take their division as a policy.  ShortDiv is the
 // short form alone, tracking the smallest and largest |m| it saw; a chunk of
 // points whose every |m| lies in the divisors' common no-rescale range is
-// exact, and any other chunk is redone with FullDiv, which matches '/'
-// always.  The common path then costs three FP64 operations and a min and
+// exact, and any other chunk (zeros, tiny or huge values, NaN-free
+// infinities) is redone with FullDiv, the division operator itself.  The common path then costs three FP64 operations and a min and
 // max per division, with no branch.
 struct ShortDiv {
   double mn = INFINITY, mx = 0.0;
@@ -145,7 +151,7 @@ struct ShortDiv {
   }
 };
 struct FullDiv {
-  __device__ __forceinline__ double operator()(double n, const Divisor& v) { return divide(n, v); }
+  __device__ __forceinline__ double operator()(double n, const Divisor& v) { return n / v.d; }
 };
 
 // build_up_b (:13-21) at interior point x.
@@ -266,11 +272,15 @@ __device__ __forceinline__ void chunk_eval(const CavityDivisors& dv, double (&ou
 }
 
 __device__ __forceinline__ CavityDivisors cavity_divisors(const CavityArgs& g) {
-  CavityDivisors dv = {uniform_divisor(g.den), uniform_divisor(g.two_dx),
-                       uniform_divisor(g.two_dy), uniform_divisor(g.dx), uniform_divisor(g.dy),
-                       0.0, 0.0};
-  dv.lo = fmax(fmax(fmax(dv.den.lo, dv.two_dx.lo), fmax(dv.two_dy.lo, dv.dx.lo)), dv.dy.lo);
-  dv.hi = fmin(fmin(fmin(dv.den.hi, dv.two_dx.hi), fmin(dv.two_dy.hi, dv.dx.hi)), dv.dy.hi);
+  double lo = 0.0, hi = INFINITY;
+  CavityDivisors dv;
+  dv.den = prepare_divisor(g.den, lo, hi);
+  dv.two_dx = prepare_divisor(g.two_dx, lo, hi);
+  dv.two_dy = prepare_divisor(g.two_dy, lo, hi);
+  dv.dx = prepare_divisor(g.dx, lo, hi);
+  dv.dy = prepare_divisor(g.dy, lo, hi);
+  dv.lo = uniform(lo);
+  dv.hi = uniform(hi);
   return dv;
 }
 
@@ -369,6 +379,7 @@ __global__ __launch_bounds__(kThreadsC) void cavity_flow_lds(CavityArgs g) {
     // in registers across the loop
 #pragma unroll
     for (int m = 0; m < kMaxPts; ++m) asm volatile("" : "+v"(pa[m]));
+    asm volatile("" : "+v"(pfl), "+v"(cnt));
     copy_velocities(g, t, np);
     double bq[kMaxPts];
 #pragma unroll
@@ -382,8 +393,12 @@ __global__ __launch_bounds__(kThreadsC) void cavity_flow_lds(CavityArgs g) {
     }
     __syncthreads();  // un and vn complete
     for (int q = 0; q < g.nit; ++q) {
+      // opaque here too, with the flags and the count: the lane masks the
+      // compiler would otherwise hoist out of this loop overflow the scalar
+      // registers
 #pragma unroll
       for (int m = 0; m < kMaxPts; ++m) asm volatile("" : "+v"(pa[m]));
+      asm volatile("" : "+v"(pfl), "+v"(cnt));
 #pragma unroll
       for (int m0 = 0; m0 < kMaxPts; m0 += kMaxPts / 2) {
         if (m0 >= cnt) continue;

@@ -57,34 +57,4 @@ __device__ __forceinline__ bool unscaled_division(double m, double lo, double hi
   return a >= lo && a < hi;
 }
 
-// A uniform divisor prepared once: its reciprocal, the quotient bounds of the
-// no-rescale condition (an empty range when the short form never applies),
-// and its sign.
-struct Divisor {
-  double d, y, lo, hi, sgn;
-  bool fast;
-};
-
-__device__ __forceinline__ Divisor make_divisor(double d) {
-  Divisor v;
-  v.d = d;
-  v.fast = den_in_range(d);
-  v.y = v.fast ? division_reciprocal(d) : 0.0;
-  quotient_bounds(fabs(d), v.lo, v.hi);
-  if (!v.fast) v.lo = INFINITY;
-  v.sgn = d > 0.0 ? 1.0 : -1.0;
-  return v;
-}
-
-// n / v.d, bit for bit, with the full sequence wherever the short form
-// cannot vouch for the result; a zero numerator over an in-range divisor is
-// the quotient's zero (n times the divisor's sign).
-__device__ __forceinline__ double divide(double n, const Divisor& v) {
-  double m;
-  const double q = short_division(n, v.d, v.y, m);
-  if (unscaled_division(m, v.lo, v.hi)) return q;
-  if (v.fast && n == 0.0) return n * v.sgn;
-  return n / v.d;
-}
-
 }  // namespace npbx
