@@ -405,6 +405,29 @@ def kernel_suite(st, dev, torch, clock, reps, copy_only=False):
             "glups": round(NT * (NIT + 2) * (NXC - 2) * (NYC - 2) * reps / ev / 1e9, 3),
             "bound": "latency (one workgroup, a barrier per phase)"}
         del c0, c1
+
+    # channel flow (FP64) at the paper preset from the reference
+    # initialisation: time steps until the reference's criterion stops them
+    # (989 steps of 53 phases); the call synchronises to read the step count
+    from npbench_amd.benchmarks.channel_flow import channel_flow as ch_init
+    NYH, NXH, NITH = 101, 101, 50
+    u, v, p, dx, dy, dt = ch_init.initialize(NYH, NXH)
+    h0 = [dev.to_device(x, torch.float64) for x in (u, v, p)]
+    h1 = [x.clone() for x in h0]
+    nsteps = []
+
+    def chf():
+        for x, y in zip(h1, h0):
+            x.copy_(y)
+        nsteps.append(st.channel_flow(NITH, *h1[:2], dt, dx, dy, h1[2], 1.0, 0.1, 1.0,
+                                      stream=clock.stream))
+    chf()
+    _, ev = clock.run(chf, reps)
+    res[f"channel_flow_paper_{NYH}x{NXH}_nit{NITH}_f64"] = {
+        "ms_per_call": round(ev / reps * 1e3, 4), "steps": nsteps[-1],
+        "us_per_phase": round(ev / reps / (nsteps[-1] * (NITH + 3)) * 1e6, 3),
+        "bound": "latency (one workgroup, a barrier per phase)"}
+    del h0, h1
     return res
 
 
@@ -462,6 +485,17 @@ def cpu_cavity_ms(sample_nt=70):
     t0 = time.perf_counter()
     ref.cavity_flow(101, 101, sample_nt, 50, u, v, dt, dx, dy, p, 1.0, 0.1)
     return round((time.perf_counter() - t0) * 1e3 * 700 / sample_nt, 1)
+
+
+def cpu_channel_ms():
+    """The NumPy oracle's channel_flow at the paper preset (101 x 101, nit=50,
+    to convergence; 1 core): milliseconds for one call."""
+    from oracle import stencils as ref
+    from npbench_amd.benchmarks.channel_flow import channel_flow as ch_init
+    u, v, p, dx, dy, dt = ch_init.initialize(101, 101)
+    t0 = time.perf_counter()
+    ref.channel_flow(50, u, v, dt, dx, dy, p, 1.0, 0.1, 1.0)
+    return round((time.perf_counter() - t0) * 1e3, 1)
 
 
 def cpu_adi_ms():
@@ -606,6 +640,9 @@ def main():
             cf["cpu_oracle_ms"] = cpu_cavity_ms()
             cf["cpu_oracle_sample"] = "nt=70 of 700 steps, scaled by 10"
             cf["speedup_vs_cpu_oracle"] = round(cf["cpu_oracle_ms"] / cf["ms_per_call"], 1)
+            ch = line["kernels"]["channel_flow_paper_101x101_nit50_f64"]
+            ch["cpu_oracle_ms"] = cpu_channel_ms()
+            ch["speedup_vs_cpu_oracle"] = round(ch["cpu_oracle_ms"] / ch["ms_per_call"], 1)
     if args.cpu_sweeps > 0:
         line["cpu_baseline"] = cpu_baseline(HEAT_N, args.cpu_sweeps)
     print(json.dumps(line), flush=True)

@@ -246,6 +246,22 @@ int npbx_cavity_flow(int dtype, int64_t nx, int64_t ny, int64_t nt, int64_t nit,
                      void *u, void *v, double dt, double dx, double dy, void *p,
                      double rho, double nu, void *stream);
 
+/* ---- channel flow (channel_flow_numpy.py:13-152) --------------------------
+ * u, v, p: (ny, nx) FP64 (dtype NPBX_F64), updated in place by time steps of
+ * nit pressure iterations each, periodic in x, until the relative change of
+ * sum(u) is at most 0.001 (the reference's while loop); the number of steps
+ * is returned in *stepcount.  The reference's arguments, plus nx, ny and
+ * max_steps, the most time steps one kernel launch may run (>= 1): launches
+ * repeat until the criterion stops one, and the call returns after a stream
+ * synchronisation.  Bitwise the NumPy result when every step's stopping
+ * decision matches NumPy's (the sums' order differs from NumPy's pairwise
+ * blocks; only a udiff within rounding of 0.001 could tell them apart).
+ * 3 <= nx, ny; nx * ny <= 2^26. */
+int npbx_channel_flow(int dtype, int64_t nit, void *u, void *v, double dt,
+                      double dx, double dy, void *p, double rho, double nu,
+                      double F, int64_t nx, int64_t ny, int64_t max_steps,
+                      int64_t *stepcount, void *stream);
+
 /* ---- alternating-direction implicit solver (adi_numpy.py:6-52) -------------
  * u: (N, N) FP64 (dtype NPBX_F64), updated in place by TSTEPS time steps of
  * the reference, each a column sweep into library scratch and a row sweep

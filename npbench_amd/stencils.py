@@ -17,6 +17,8 @@ Two layers:
 Every call goes to libnpbx.so; there is no CPU path.
 """
 
+import ctypes
+
 import numpy as np
 import torch
 
@@ -121,6 +123,23 @@ def cavity_flow(nx, ny, nt, nit, u, v, dt, dx, dy, p, rho, nu, stream=None):
     nat.call("npbx_cavity_flow", dev.dtype_code(u), int(nx), int(ny), int(nt), int(nit),
              dev.ptr(u), dev.ptr(v), float(dt), float(dx), float(dy), dev.ptr(p), float(rho),
              float(nu), dev.stream_handle(stream))
+
+
+def channel_flow(nit, u, v, dt, dx, dy, p, rho, nu, F, stream=None, max_steps=None):
+    """channel_flow_numpy.channel_flow on float64 (ny, nx) device tensors u,
+    v, p (in place); returns the step count.  Synchronises the stream (the
+    number of steps depends on the data).  max_steps bounds the steps of one
+    kernel launch; by default a launch holds about 50 ms of work."""
+    ny, nx = (int(d) for d in u.shape)
+    for name, t in (("u", u), ("v", v), ("p", p)):
+        dev.check_device_array(name, t, shape=(ny, nx), dtype=torch.float64)
+    if max_steps is None:
+        max_steps = max(1, min(4096, 150_000_000 // (nx * ny * (int(nit) + 3))))
+    steps = ctypes.c_int64(0)
+    nat.call("npbx_channel_flow", dev.dtype_code(u), int(nit), dev.ptr(u), dev.ptr(v), float(dt),
+             float(dx), float(dy), dev.ptr(p), float(rho), float(nu), float(F), nx, ny,
+             int(max_steps), ctypes.byref(steps), dev.stream_handle(stream))
+    return steps.value
 
 
 def cast_into(src, dst, stream=None):
@@ -234,6 +253,13 @@ def fdtd_2d(TMAX, ex, ey, hz, _fict_):
     place."""
     with _Staged("fp64", [ex, ey, hz, _fict_], outputs=(0, 1, 2)) as (a, b, c, f):
         fdtd2d(a, b, c, f, TMAX)
+
+
+def channel_flow_dropin(nit, u, v, dt, dx, dy, p, rho, nu, F):
+    """channel_flow_numpy.channel_flow(...): u, v and p updated in place; the
+    step count returned."""
+    with _Staged("fp64", [u, v, p], outputs=(0, 1, 2)) as (a, b, c):
+        return channel_flow(nit, a, b, dt, dx, dy, c, rho, nu, F)
 
 
 def cavity_flow_dropin(nx, ny, nt, nit, u, v, dt, dx, dy, p, rho, nu):

@@ -319,3 +319,61 @@ def cavity_flow(nx, ny, nt, nit, u, v, dt, dx, dy, p, rho, nu):
             V[i][0] = 0.0
             V[i][nx - 1] = 0.0
     return np.array(U), np.array(V), np.array(P)
+
+
+def channel_flow(nit, u, v, dt, dx, dy, p, rho, nu, F):
+    """channel_flow_numpy.py:13-152, one point at a time (x-neighbours wrap
+    around); the stopping test uses np.sum as the reference does.  Returns
+    new (u, v, p, stepcount)."""
+    ny, nx = u.shape
+    U = u.tolist()
+    V = v.tolist()
+    P = p.tolist()
+    E = [(j + 1) % nx for j in range(nx)]
+    W = [(j - 1) % nx for j in range(nx)]
+    udiff, steps = 1, 0
+    while udiff > .001:
+        un = [r[:] for r in U]
+        vn = [r[:] for r in V]
+        B = [[0.0] * nx for _ in range(ny)]
+        for i in range(1, ny - 1):
+            for j in range(nx):
+                e, w = E[j], W[j]
+                dudx = (U[i][e] - U[i][w]) / (2 * dx)
+                dvdy = (V[i + 1][j] - V[i - 1][j]) / (2 * dy)
+                cross = (U[i + 1][j] - U[i - 1][j]) / (2 * dy) * (V[i][e] - V[i][w]) / (2 * dx)
+                B[i][j] = rho * ((((1 / dt) * (dudx + dvdy) - dudx * dudx) - 2 * cross)
+                                 - dvdy * dvdy)
+        d2x, d2y = dx ** 2, dy ** 2
+        den = 2 * (d2x + d2y)
+        for _q in range(nit):
+            pn = [r[:] for r in P]
+            for i in range(1, ny - 1):
+                for j in range(nx):
+                    num = (pn[i][E[j]] + pn[i][W[j]]) * d2y + (pn[i + 1][j] + pn[i - 1][j]) * d2x
+                    P[i][j] = num / den - (d2x * d2y / den) * B[i][j]
+            P[ny - 1] = P[ny - 2][:]
+            P[0] = P[1][:]
+        for i in range(1, ny - 1):
+            for j in range(nx):
+                e, w = E[j], W[j]
+                uc, vc = un[i][j], vn[i][j]
+                x = uc - ((uc * dt) / dx) * (uc - un[i][w])
+                x = x - ((vc * dt) / dy) * (uc - un[i - 1][j])
+                x = x - (dt / (2 * rho * dx)) * (P[i][e] - P[i][w])
+                lap = (dt / dx ** 2) * ((un[i][e] - 2 * uc) + un[i][w])
+                lap = lap + (dt / dy ** 2) * ((un[i + 1][j] - 2 * uc) + un[i - 1][j])
+                U[i][j] = (x + nu * lap) + F * dt
+                y = vc - ((uc * dt) / dx) * (vc - vn[i][w])
+                y = y - ((vc * dt) / dy) * (vc - vn[i - 1][j])
+                y = y - (dt / (2 * rho * dy)) * (P[i + 1][j] - P[i - 1][j])
+                lap = (dt / dx ** 2) * ((vn[i][e] - 2 * vc) + vn[i][w])
+                lap = lap + (dt / dy ** 2) * ((vn[i + 1][j] - 2 * vc) + vn[i - 1][j])
+                V[i][j] = y + nu * lap
+        for j in range(nx):
+            U[0][j] = U[ny - 1][j] = 0.0
+            V[0][j] = V[ny - 1][j] = 0.0
+        su, sun = np.sum(np.array(U)), np.sum(np.array(un))
+        udiff = (su - sun) / su
+        steps += 1
+    return np.array(U), np.array(V), np.array(P), steps

@@ -13,6 +13,7 @@ Reference files (paths relative to the npbench source tree):
   jacobi_1d_kernel  npbench/benchmarks/polybench/jacobi_1d/jacobi_1d_numpy.py:4-8
   fdtd_2d_kernel    npbench/benchmarks/polybench/fdtd_2d/fdtd_2d_numpy.py:4-11
   cavity_flow       npbench/benchmarks/cavity_flow/cavity_flow_numpy.py:13-83
+  channel_flow      npbench/benchmarks/channel_flow/channel_flow_numpy.py:13-152
 """
 
 import numpy as np
@@ -411,3 +412,65 @@ def cavity_flow(nx, ny, nt, nit, u, v, dt, dx, dy, p, rho, nu):
         v[-1, :] = 0
         v[:, 0] = 0
         v[:, -1] = 0
+
+
+# --------------------------------------------------------------------------
+# Channel flow (periodic in x, walls at rows 0 and ny-1)
+# --------------------------------------------------------------------------
+def _channel_b(rho, dt, dx, dy, u, v):
+    """build_up_b (channel_flow_numpy.py:13-39): the interior as in the
+    cavity, plus the periodic columns 0 and nx-1, whose x-neighbours wrap."""
+    ny, nx = u.shape
+    b = np.zeros_like(u)
+    for j in range(nx):
+        e, w = (j + 1) % nx, (j - 1) % nx
+        dudx = (u[1:-1, e] - u[1:-1, w]) / (2 * dx)
+        dvdy = (v[2:, j] - v[:-2, j]) / (2 * dy)
+        cross = (u[2:, j] - u[:-2, j]) / (2 * dy) * (v[1:-1, e] - v[1:-1, w]) / (2 * dx)
+        b[1:-1, j] = rho * (1 / dt * (dudx + dvdy) - dudx ** 2 - 2 * cross - dvdy ** 2)
+    return b
+
+
+def _channel_pressure(nit, p, dx, dy, b):
+    """pressure_poisson_periodic (channel_flow_numpy.py:42-66)."""
+    ny, nx = p.shape
+    den = 2 * (dx ** 2 + dy ** 2)
+    e = np.r_[1:nx, 0]
+    w = np.r_[nx - 1, 0:nx - 1]
+    for _ in range(nit):
+        pn = p.copy()
+        p[1:-1, :] = (((pn[1:-1, e] + pn[1:-1, w]) * dy ** 2 + (pn[2:, :] + pn[:-2, :]) * dx ** 2)
+                      / den - dx ** 2 * dy ** 2 / den * b[1:-1, :])
+        p[-1, :] = p[-2, :]
+        p[0, :] = p[1, :]
+
+
+def channel_flow(nit, u, v, dt, dx, dy, p, rho, nu, F):
+    """channel_flow_numpy.channel_flow: time steps until the relative change
+    of sum(u) is at most 0.001; u, v, p in place; returns the step count."""
+    _f64(u, v, p)
+    ny, nx = u.shape
+    e = np.r_[1:nx, 0]
+    w = np.r_[nx - 1, 0:nx - 1]
+    udiff, stepcount = 1, 0
+    while udiff > .001:
+        un = u.copy()
+        vn = v.copy()
+        b = _channel_b(rho, dt, dx, dy, u, v)
+        _channel_pressure(nit, p, dx, dy, b)
+        uc, vc = un[1:-1, :], vn[1:-1, :]
+        u[1:-1, :] = (uc - uc * dt / dx * (uc - un[1:-1, w]) - vc * dt / dy * (uc - un[:-2, :])
+                      - dt / (2 * rho * dx) * (p[1:-1, e] - p[1:-1, w])
+                      + nu * (dt / dx ** 2 * (un[1:-1, e] - 2 * uc + un[1:-1, w])
+                              + dt / dy ** 2 * (un[2:, :] - 2 * uc + un[:-2, :])) + F * dt)
+        v[1:-1, :] = (vc - uc * dt / dx * (vc - vn[1:-1, w]) - vc * dt / dy * (vc - vn[:-2, :])
+                      - dt / (2 * rho * dy) * (p[2:, :] - p[:-2, :])
+                      + nu * (dt / dx ** 2 * (vn[1:-1, e] - 2 * vc + vn[1:-1, w])
+                              + dt / dy ** 2 * (vn[2:, :] - 2 * vc + vn[:-2, :])))
+        u[0, :] = 0
+        u[-1, :] = 0
+        v[0, :] = 0
+        v[-1, :] = 0
+        udiff = (np.sum(u) - np.sum(un)) / np.sum(u)
+        stepcount += 1
+    return stepcount

@@ -23,6 +23,8 @@ Cases (all FP64):
   fdtd_2d     (NX, NY) = (5, 6) TMAX=3 reference init; (4, 7) TMAX=4 seeded
   cavity_flow (ny, nx) = (6, 8) nt=4 nit=3 reference init (zeros, the lid
               drives the flow); (5, 7) nt=3 nit=4 seeded u, v, p
+  channel_flow (ny, nx) = (4, 5) nit=2 reference init (u = v = 0, p = 1;
+              converges in 564 steps); (5, 6) nit=3 from rest with seeded v, p
 
 Arguments, if given, are name prefixes: only the matching files are written.
 """
@@ -45,6 +47,7 @@ from npbench_amd.benchmarks.polybench.adi import adi  # noqa: E402
 from npbench_amd.benchmarks.polybench.jacobi_1d import jacobi_1d  # noqa: E402
 from npbench_amd.benchmarks.polybench.fdtd_2d import fdtd_2d  # noqa: E402
 from npbench_amd.benchmarks.cavity_flow import cavity_flow  # noqa: E402
+from npbench_amd.benchmarks.channel_flow import channel_flow  # noqa: E402
 
 ONLY = tuple(sys.argv[1:])
 
@@ -127,6 +130,16 @@ def main():
         ou, ov, op = scalar.cavity_flow(nx, ny, nt, nit, u, v, dt, dx, dy, p, 1.0, 0.1)
         save(f"cavity_flow_{tag}", nx=nx, ny=ny, nt=nt, nit=nit, dt=dt, dx=dx, dy=dy, rho=1.0,
              nu=0.1, u=u, v=v, p=p, u_out=ou, v_out=ov, p_out=op)
+
+    # channel flow (runs to the reference's stopping criterion)
+    rng = np.random.default_rng(23)
+    for tag, (ny, nx, nit), seeded in (("ref", (4, 5, 2), False), ("uniform", (5, 6, 3), True)):
+        u, v, p, dx, dy, dt = channel_flow.initialize(ny, nx)
+        if seeded:
+            v, p = 0.01 * rng.random((ny, nx)), 1 + 0.1 * rng.random((ny, nx))
+        ou, ov, op, steps = scalar.channel_flow(nit, u, v, dt, dx, dy, p, 1.0, 0.1, 1.0)
+        save(f"channel_flow_{tag}", nit=nit, dt=dt, dx=dx, dy=dy, rho=1.0, nu=0.1, F=1.0, u=u, v=v,
+             p=p, u_out=ou, v_out=ov, p_out=op, stepcount=steps)
 
 
 if __name__ == "__main__":

@@ -70,6 +70,16 @@ def test_version_string():
                       0.1, None), "negative"),
     (lambda: nat.call("npbx_cavity_flow", 0, 5, 5, 2, 2, None, None, 0.1, 0.5, 0.5, None, 1.0,
                       0.1, None), "distinct"),
+    (lambda: nat.call("npbx_channel_flow", 1, 2, None, None, 0.1, 0.5, 0.5, None, 1.0, 0.1, 1.0,
+                      5, 5, 10, ctypes.byref(ctypes.c_int64()), None), "dtype"),
+    (lambda: nat.call("npbx_channel_flow", 0, 2, None, None, 0.1, 0.5, 0.5, None, 1.0, 0.1, 1.0,
+                      2, 5, 10, ctypes.byref(ctypes.c_int64()), None), "out of range"),
+    (lambda: nat.call("npbx_channel_flow", 0, 2, None, None, 0.1, 0.5, 0.5, None, 1.0, 0.1, 1.0,
+                      5, 5, 0, ctypes.byref(ctypes.c_int64()), None), "max_steps"),
+    (lambda: nat.call("npbx_channel_flow", 0, 2, None, None, 0.1, 0.5, 0.5, None, 1.0, 0.1, 1.0,
+                      5, 5, 10, None, None), "stepcount"),
+    (lambda: nat.call("npbx_channel_flow", 0, 2, None, None, 0.1, 0.5, 0.5, None, 1.0, 0.1, 1.0,
+                      5, 5, 10, ctypes.byref(ctypes.c_int64()), None), "distinct"),
 ])
 def test_argument_errors_raise_with_message(call, needle):
     with pytest.raises(nat.NpbxError) as ei:

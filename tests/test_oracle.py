@@ -43,6 +43,12 @@ fails at least one check:
              source term's closed forms (the divergence and the cross term)
              through p = -dx^2 dy^2/(2(dx^2 + dy^2)) b; linear fields advect
              by u du/dx and v dv/dy; brute force and golden vectors.
+  channel    from rest with p = 1 the first step gives u = F dt on every
+             row between the walls and nothing else (the force term; p stays
+             1); a field rolled along x gives the rolled result with the same
+             step count (the periodic columns wrap, no column is special);
+             brute force (to convergence, hundreds of steps) and golden
+             vectors with their step counts.
 
 The last section plants mistakes in the oracle (monkeypatched copies) and
 checks the pins above catch them.
@@ -759,6 +765,97 @@ def test_planted_cavity_mistakes_are_caught(monkeypatch):
     assert all(pin() is None for pin in CAVITY_PINS)
 
 
+# ------------------------------------------------------------------ channel flow
+def channel_run(ny, nx, nit, u=None, v=None, p=None, F=1.0):
+    """S.channel_flow (looked up per call) on the reference grid spacing;
+    returns (u, v, p, steps, dt)."""
+    dx, dy, dt = 2 / (nx - 1), 2 / (ny - 1), .1 / ((nx - 1) * (ny - 1))
+    u = np.zeros((ny, nx)) if u is None else u.copy()
+    v = np.zeros((ny, nx)) if v is None else v.copy()
+    p = np.ones((ny, nx)) if p is None else p.copy()
+    steps = S.channel_flow(nit, u, v, dt, dx, dy, p, 1.0, 0.1, F)
+    return u, v, p, steps, dt
+
+
+def test_channel_first_step_is_the_force():
+    """With u = v = 0 and p = 1, step one has no source, no gradient and no
+    advection: u = F dt between the walls, v = 0, p = 1.  The stopping test
+    then sees udiff = 1, so a second step runs; a zero-F run stops at once
+    (0/0 is NaN, which ends the loop)."""
+    ny, nx, F = 5, 7, 2.5
+    dx, dy, dt = 2 / (nx - 1), 2 / (ny - 1), .1 / ((nx - 1) * (ny - 1))
+    u, v, p = np.zeros((ny, nx)), np.zeros((ny, nx)), np.ones((ny, nx))
+    calls = []
+    real_b = S._channel_b
+
+    def spy(*a):
+        calls.append(a[4].copy())
+        return real_b(*a)
+    S._channel_b = spy
+    try:
+        steps = S.channel_flow(3, u, v, dt, dx, dy, p, 1.0, 0.1, F)
+    finally:
+        S._channel_b = real_b
+    assert steps >= 2
+    assert np.all(calls[1][1:-1] == F * dt) and not calls[1][[0, -1]].any()
+    with np.errstate(invalid="ignore"):
+        u, v, p, steps, _ = channel_run(ny, nx, 3, F=0.0)
+    assert steps == 1 and not u.any() and not v.any() and np.all(p == 1.0)
+
+
+def test_channel_is_periodic_in_x():
+    """Rolling the input along x rolls the output: every column, 0 and nx-1
+    included, is computed with wrapped neighbours.  (Bit-exact per point;
+    the step count is compared too.)"""
+    ny, nx, nit = 5, 8, 2
+    rng = np.random.default_rng(4)
+    v0, p0 = 0.01 * rng.random((ny, nx)), 1 + 0.1 * rng.random((ny, nx))
+    u, v, p, steps, _ = channel_run(ny, nx, nit, v=v0, p=p0)
+    for k in (1, 3):
+        ur, vr, pr, sr, _ = channel_run(ny, nx, nit, v=np.roll(v0, k, 1), p=np.roll(p0, k, 1))
+        assert sr == steps
+        assert np.allclose(ur, np.roll(u, k, 1), rtol=1e-12, atol=1e-15)
+        assert np.allclose(vr, np.roll(v, k, 1), rtol=1e-12, atol=1e-15)
+        assert np.allclose(pr, np.roll(p, k, 1), rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("ny,nx,nit", [(3, 4, 1), (4, 5, 2), (5, 3, 3), (6, 7, 4)])
+def test_channel_bruteforce_bitwise(ny, nx, nit):
+    rng = np.random.default_rng(ny * 41 + nx)
+    v0, p0 = 0.01 * rng.random((ny, nx)), 1 + 0.1 * rng.random((ny, nx))
+    u, v, p, steps, dt = channel_run(ny, nx, nit, v=v0, p=p0)
+    dx, dy = 2 / (nx - 1), 2 / (ny - 1)
+    ru, rv, rp, rs = scalar.channel_flow(nit, np.zeros((ny, nx)), v0, dt, dx, dy, p0, 1.0, 0.1, 1.0)
+    assert steps == rs and steps > 100
+    assert np.array_equal(u, ru) and np.array_equal(v, rv) and np.array_equal(p, rp)
+
+
+def test_planted_channel_mistakes_are_caught(monkeypatch):
+    """A dropped force term and a non-periodic (clamped) x-neighbour each
+    fail a pin."""
+    good = S.channel_flow
+
+    def no_force(nit, u, v, dt, dx, dy, p, rho, nu, F):
+        return good(nit, u, v, dt, dx, dy, p, rho, nu, 0.0 * F + 1e-300)
+
+    def clamped(nit, u, v, dt, dx, dy, p, rho, nu, F):
+        # pad with copies of the edge columns instead of wrapping: columns
+        # 0 and nx-1 then see themselves as their outer neighbour
+        ny, nx = u.shape
+        big = [np.concatenate([a[:, :1], a, a[:, -1:]], axis=1) for a in (u, v, p)]
+        steps = good(nit, *big[:2], dt, dx, dy, big[2], rho, nu, F)
+        u[...], v[...], p[...] = big[0][:, 1:-1], big[1][:, 1:-1], big[2][:, 1:-1]
+        return steps
+
+    for bad in (no_force, clamped):
+        with monkeypatch.context() as m:
+            m.setattr(S, "channel_flow", bad)
+            with pytest.raises(AssertionError):
+                with np.errstate(all="ignore"):
+                    test_channel_first_step_is_the_force()
+                    test_channel_is_periodic_in_x()
+
+
 # ------------------------------------------------------------------ golden
 def _golden(name):
     with np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False) as z:
@@ -817,6 +914,17 @@ def test_golden_cavity_flow(name):
     u, v, p = g["u"].copy(), g["v"].copy(), g["p"].copy()
     S.cavity_flow(int(g["nx"]), int(g["ny"]), int(g["nt"]), int(g["nit"]), u, v, float(g["dt"]),
                   float(g["dx"]), float(g["dy"]), p, float(g["rho"]), float(g["nu"]))
+    assert np.array_equal(u, g["u_out"]) and np.array_equal(v, g["v_out"])
+    assert np.array_equal(p, g["p_out"])
+
+
+@pytest.mark.parametrize("name", ["channel_flow_ref", "channel_flow_uniform"])
+def test_golden_channel_flow(name):
+    g = _golden(name)
+    u, v, p = g["u"].copy(), g["v"].copy(), g["p"].copy()
+    steps = S.channel_flow(int(g["nit"]), u, v, float(g["dt"]), float(g["dx"]), float(g["dy"]), p,
+                           float(g["rho"]), float(g["nu"]), float(g["F"]))
+    assert steps == int(g["stepcount"])
     assert np.array_equal(u, g["u_out"]) and np.array_equal(v, g["v_out"])
     assert np.array_equal(p, g["p_out"])
 
