@@ -18,7 +18,8 @@ from .infrastructure import Benchmark, Test, generate_framework
 from .infrastructure import utilities as util
 
 # The stencil path this build implements (bench_info/*.json).
-BENCHMARKS = ("jacobi_2d", "heat_3d", "hdiff", "vadv")
+BENCHMARKS = ("jacobi_2d", "heat_3d", "hdiff", "vadv", "adi", "jacobi_1d", "fdtd_2d",
+              "cavity_flow", "channel_flow")
 
 
 def parse(argv=None):
