@@ -132,6 +132,9 @@ def lib():
             if not LIB_PATH.exists():
                 raise NativeLibraryMissing(
                     f"{LIB_PATH} is missing: build it with `python -m npbench_amd.build`")
+            # torch's HIP runtime first, so that libnpbx.so binds to the one
+            # the process's tensors use, whichever module asks for it first
+            import torch  # noqa: F401
             handle = ctypes.CDLL(str(LIB_PATH))
             for name, argtypes in SIGNATURES.items():
                 fn = getattr(handle, name)
