@@ -62,31 +62,12 @@ struct CavityArgs {
 
 // The kernel's divisors, prepared once per launch and held in scalar
 // registers: every division of the time step is by one of them.  Only their
-// reciprocals and one common no-rescale range of |m| stay live (an empty
-// range when a divisor is outside den_in_range), so the kernel's uniform
-// values fit the scalar register file.
-__device__ __forceinline__ double uniform(double x) {
-  return __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(x)),
-                          __builtin_amdgcn_readfirstlane(__double2loint(x)));
-}
-
-struct Divisor {
-  double d, y;
-};
-
+// reciprocals and one common no-rescale range of |m| stay live, so the
+// kernel's uniform values fit the scalar register file.
 struct CavityDivisors {
   Divisor den, two_dx, two_dy, dx, dy;
   double lo, hi;  // |m| in [lo, hi) for every division of a chunk: exact
 };
-
-__device__ __forceinline__ Divisor prepare_divisor(double d, double& lo, double& hi) {
-  double dlo, dhi;
-  quotient_bounds(fabs(d), dlo, dhi);
-  const bool fast = den_in_range(d);
-  lo = fmax(lo, fast ? dlo : INFINITY);
-  hi = fmin(hi, dhi);
-  return {d, uniform(fast ? division_reciprocal(d) : 0.0)};
-}
 
 // Walks this thread's interior points k = t, t + kThreadsC, ... in row-major
 // order of the (ny-2) x (nx-2) interior without a division per point, C
@@ -134,25 +115,8 @@ constexpr int kChunkB = 4;  // source points per chunk (8 loads each)
 constexpr int kChunkV = 2;  // velocity points per chunk (14 loads each)
 constexpr int kMaxPts = kLdsPoints / kThreadsC;  // interior points per thread, LDS grids
 
-// The point computations take their division as a policy.  ShortDiv is the
-// short form alone, tracking the smallest and largest |m| it saw; a chunk of
-// points whose every |m| lies in the divisors' common no-rescale range is
-// exact, and any other chunk (zeros, tiny or huge values, NaN-free
-// infinities) is redone with FullDiv, the division operator itself.  The common path then costs three FP64 operations and a min and
-// max per division, with no branch.
-struct ShortDiv {
-  double mn = INFINITY, mx = 0.0;
-  __device__ __forceinline__ double operator()(double n, const Divisor& v) {
-    double m;
-    const double q = short_division(n, v.d, v.y, m);
-    mn = fmin(mn, fabs(m));
-    mx = fmax(mx, fabs(m));
-    return q;
-  }
-};
-struct FullDiv {
-  __device__ __forceinline__ double operator()(double n, const Divisor& v) { return n / v.d; }
-};
+// The point computations take their division as a policy (ShortDiv or
+// FullDiv, run by chunk_eval).
 
 // build_up_b (:13-21) at interior point x.
 template <class Div>
@@ -257,20 +221,6 @@ __device__ __forceinline__ void copy_velocities(const CavityArgs& g, int t, int 
   }
 }
 
-// Fills out[c] = f(c, div) for c < K with the short division, and redoes
-// the chunk with the full one in the lanes where it could not vouch.
-template <int K, class F>
-__device__ __forceinline__ void chunk_eval(const CavityDivisors& dv, double (&out)[K], F&& f) {
-  ShortDiv sd;
-#pragma unroll
-  for (int c = 0; c < K; ++c) out[c] = f(c, sd);
-  if (!(sd.mn >= dv.lo && sd.mx < dv.hi)) {
-    FullDiv fd;
-#pragma unroll
-    for (int c = 0; c < K; ++c) out[c] = f(c, fd);
-  }
-}
-
 __device__ __forceinline__ CavityDivisors cavity_divisors(const CavityArgs& g) {
   double lo = 0.0, hi = INFINITY;
   CavityDivisors dv;
@@ -279,8 +229,8 @@ __device__ __forceinline__ CavityDivisors cavity_divisors(const CavityArgs& g) {
   dv.two_dy = prepare_divisor(g.two_dy, lo, hi);
   dv.dx = prepare_divisor(g.dx, lo, hi);
   dv.dy = prepare_divisor(g.dy, lo, hi);
-  dv.lo = uniform(lo);
-  dv.hi = uniform(hi);
+  dv.lo = uniform_double(lo);
+  dv.hi = uniform_double(hi);
   return dv;
 }
 

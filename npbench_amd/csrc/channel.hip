@@ -52,27 +52,10 @@ struct ChannelArgs {
   double fdt;  // F * dt
 };
 
-__device__ __forceinline__ double uniform(double x) {
-  return __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(x)),
-                          __builtin_amdgcn_readfirstlane(__double2loint(x)));
-}
-
-struct Divisor {
-  double d, y;
-};
 struct Divisors {
   Divisor den, two_dx, two_dy, dx, dy;
   double lo, hi;
 };
-
-__device__ __forceinline__ Divisor prepare_divisor(double d, double& lo, double& hi) {
-  double dlo, dhi;
-  quotient_bounds(fabs(d), dlo, dhi);
-  const bool fast = den_in_range(d);
-  lo = fmax(lo, fast ? dlo : INFINITY);
-  hi = fmin(hi, dhi);
-  return {d, uniform(fast ? division_reciprocal(d) : 0.0)};
-}
 
 __device__ __forceinline__ Divisors prepare_divisors(const ChannelArgs& g) {
   double lo = 0.0, hi = INFINITY;
@@ -82,37 +65,9 @@ __device__ __forceinline__ Divisors prepare_divisors(const ChannelArgs& g) {
   dv.two_dy = prepare_divisor(g.two_dy, lo, hi);
   dv.dx = prepare_divisor(g.dx, lo, hi);
   dv.dy = prepare_divisor(g.dy, lo, hi);
-  dv.lo = uniform(lo);
-  dv.hi = uniform(hi);
+  dv.lo = uniform_double(lo);
+  dv.hi = uniform_double(hi);
   return dv;
-}
-
-// The short division with min/max tracking, and the division operator (a
-// chunk outside the exact range is redone with it).
-struct ShortDiv {
-  double mn = INFINITY, mx = 0.0;
-  __device__ __forceinline__ double operator()(double n, const Divisor& v) {
-    double m;
-    const double q = short_division(n, v.d, v.y, m);
-    mn = fmin(mn, fabs(m));
-    mx = fmax(mx, fabs(m));
-    return q;
-  }
-};
-struct FullDiv {
-  __device__ __forceinline__ double operator()(double n, const Divisor& v) { return n / v.d; }
-};
-
-template <int K, class F>
-__device__ __forceinline__ void chunk_eval(const Divisors& dv, double (&out)[K], F&& f) {
-  ShortDiv sd;
-#pragma unroll
-  for (int c = 0; c < K; ++c) out[c] = f(c, sd);
-  if (!(sd.mn >= dv.lo && sd.mx < dv.hi)) {
-    FullDiv fd;
-#pragma unroll
-    for (int c = 0; c < K; ++c) out[c] = f(c, fd);
-  }
 }
 
 // A point of rows 1..ny-2, packed in one int: its flat index x (< 2^26)

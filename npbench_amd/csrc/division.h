@@ -57,4 +57,59 @@ __device__ __forceinline__ bool unscaled_division(double m, double lo, double hi
   return a >= lo && a < hi;
 }
 
+// A wave-uniform double (the value of the first active lane), so that values
+// every lane computes alike live in scalar registers.
+__device__ __forceinline__ double uniform_double(double x) {
+  return __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(x)),
+                          __builtin_amdgcn_readfirstlane(__double2loint(x)));
+}
+
+// A uniform divisor and its reciprocal.  prepare_divisor narrows [lo, hi),
+// the |m| range in which the short form is exact for every divisor prepared
+// into it (empty when a divisor is outside den_in_range).
+struct Divisor {
+  double d, y;
+};
+
+__device__ __forceinline__ Divisor prepare_divisor(double d, double& lo, double& hi) {
+  double dlo, dhi;
+  quotient_bounds(fabs(d), dlo, dhi);
+  const bool fast = den_in_range(d);
+  lo = fmax(lo, fast ? dlo : INFINITY);
+  hi = fmin(hi, dhi);
+  return {d, uniform_double(fast ? division_reciprocal(d) : 0.0)};
+}
+
+// Division policies for code that computes a chunk of quotients at once.
+// ShortDiv is the short form alone and tracks the smallest and largest |m|
+// it saw; FullDiv is the division operator.
+struct ShortDiv {
+  double mn = INFINITY, mx = 0.0;
+  __device__ __forceinline__ double operator()(double n, const Divisor& v) {
+    double m;
+    const double q = short_division(n, v.d, v.y, m);
+    mn = fmin(mn, fabs(m));
+    mx = fmax(mx, fabs(m));
+    return q;
+  }
+};
+struct FullDiv {
+  __device__ __forceinline__ double operator()(double n, const Divisor& v) { return n / v.d; }
+};
+
+// out[c] = f(c, div) for c < K with the short division; a chunk whose |m|
+// left [dv.lo, dv.hi) anywhere is redone with the division operator.  With
+// no branch on the common path the chunk's loads issue together.
+template <int K, class D, class F>
+__device__ __forceinline__ void chunk_eval(const D& dv, double (&out)[K], F&& f) {
+  ShortDiv sd;
+#pragma unroll
+  for (int c = 0; c < K; ++c) out[c] = f(c, sd);
+  if (!(sd.mn >= dv.lo && sd.mx < dv.hi)) {
+    FullDiv fd;
+#pragma unroll
+    for (int c = 0; c < K; ++c) out[c] = f(c, fd);
+  }
+}
+
 }  // namespace npbx
