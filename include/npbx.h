@@ -239,8 +239,9 @@ int npbx_fdtd2d(int dtype, void *ex, void *ey, void *hz, const void *fict,
  * u, v, p: (ny, nx) FP64 (dtype NPBX_F64), updated in place by nt time steps
  * of nit pressure iterations each, with the reference's argument order and
  * scalars (cavity_flow_numpy.py:41); bitwise the NumPy result.  3 <= nx, ny
- * and nx * ny <= 2^26; nt == 0 is a no-op.  One workgroup runs the whole
- * call; grids of up to 10240 points keep the pressure in LDS.  A device
+ * and nx * ny <= 2^26; nt == 0 is a no-op.  One workgroup runs the steps,
+ * in launches of about 50 ms of work each; grids of up to 10240 points keep
+ * the pressure in LDS.  A device
  * workspace of 4 nx ny doubles is reused between calls. */
 int npbx_cavity_flow(int dtype, int64_t nx, int64_t ny, int64_t nt, int64_t nit,
                      void *u, void *v, double dt, double dx, double dy, void *p,

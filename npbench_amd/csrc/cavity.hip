@@ -24,6 +24,7 @@
 // an iteration is a single phase.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "common.h"
@@ -439,12 +440,19 @@ extern "C" int npbx_cavity_flow(int dtype, int64_t nx, int64_t ny, int64_t nt, i
   g.cpv = dt / (2.0 * rho * dy);
   g.cdx2 = dt / g.dx2;
   g.cdy2 = dt / g.dy2;
-  if (np <= kLdsPoints) {
-    hipLaunchKernelGGL(cavity_flow_lds, dim3(1), dim3(kThreadsC), 0, as_stream(stream), g);
-    NPBX_LAUNCHED("cavity_flow_lds");
-  } else {
-    hipLaunchKernelGGL(cavity_flow_global, dim3(1), dim3(kThreadsC), 0, as_stream(stream), g);
-    NPBX_LAUNCHED("cavity_flow_global");
+  // A time step depends on u, v and p only (b, un, vn are rebuilt each step),
+  // so the steps split into launches of about 50 ms of work each: no launch
+  // runs for long, whatever nt is.
+  const int64_t per_launch = std::max<int64_t>(1, int64_t(150000000) / (np * (nit + 3)));
+  for (int64_t done = 0; done < nt; done += per_launch) {
+    g.nt = static_cast<int>(std::min<int64_t>(per_launch, nt - done));
+    if (np <= kLdsPoints) {
+      hipLaunchKernelGGL(cavity_flow_lds, dim3(1), dim3(kThreadsC), 0, as_stream(stream), g);
+      NPBX_LAUNCHED("cavity_flow_lds");
+    } else {
+      hipLaunchKernelGGL(cavity_flow_global, dim3(1), dim3(kThreadsC), 0, as_stream(stream), g);
+      NPBX_LAUNCHED("cavity_flow_global");
+    }
   }
   return NPBX_OK;
 }

@@ -67,6 +67,14 @@ def test_cavity_reference_presets_bitwise(st, ny, nx, nt, nit):
     check(st, nx, ny, nt, nit, u, v, p, 1.0, 0.1, f"preset {ny}x{nx} nt={nt} nit={nit}")
 
 
+def test_cavity_paper_preset_bitwise(st):
+    """The paper preset (101 x 101, 700 steps of 50 iterations) runs as
+    three launches of at most ~280 steps: the split is invisible."""
+    from npbench_amd.benchmarks.cavity_flow import cavity_flow
+    u, v, p, dx, dy, dt = cavity_flow.initialize(101, 101)
+    check(st, 101, 101, 700, 50, u, v, p, 1.0, 0.1, "preset paper")
+
+
 def test_cavity_wide_grid(st):
     """An interior row longer than the workgroup (the walk's row stride is 0)."""
     rng = np.random.default_rng(5)
