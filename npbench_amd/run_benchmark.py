@@ -19,7 +19,7 @@ from .infrastructure import utilities as util
 
 # The stencil path this build implements (bench_info/*.json).
 BENCHMARKS = ("jacobi_2d", "heat_3d", "hdiff", "vadv", "adi", "jacobi_1d", "fdtd_2d",
-              "cavity_flow", "channel_flow")
+              "cavity_flow", "channel_flow", "seidel_2d")
 
 
 def parse(argv=None):

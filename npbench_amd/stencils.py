@@ -142,6 +142,16 @@ def channel_flow(nit, u, v, dt, dx, dy, p, rho, nu, F, stream=None, max_steps=No
     return steps.value
 
 
+def seidel2d(A, TSTEPS, stream=None):
+    """seidel_2d_numpy.kernel's TSTEPS - 1 sweeps on a float64 (N, N) device
+    tensor, in place."""
+    dev.check_device_array("A", A, dtype=torch.float64)
+    if A.dim() != 2 or A.shape[0] != A.shape[1]:
+        raise ValueError(f"A must be square (N, N), got {tuple(A.shape)}")
+    nat.call("npbx_seidel2d", dev.dtype_code(A), dev.ptr(A), int(A.shape[0]), int(TSTEPS),
+             dev.stream_handle(stream))
+
+
 def cast_into(src, dst, stream=None):
     """dst[...] = src converted to dst's dtype (RNE), on the device."""
     dev.check_device_array("src", src)
@@ -253,6 +263,14 @@ def fdtd_2d(TMAX, ex, ey, hz, _fict_):
     place."""
     with _Staged("fp64", [ex, ey, hz, _fict_], outputs=(0, 1, 2)) as (a, b, c, f):
         fdtd2d(a, b, c, f, TMAX)
+
+
+def seidel_2d(TSTEPS, N, A):
+    """seidel_2d_numpy.kernel(TSTEPS, N, A): A updated in place."""
+    if tuple(A.shape) != (int(N), int(N)):
+        raise ValueError(f"A has shape {tuple(A.shape)}, expected ({N}, {N})")
+    with _Staged("fp64", [A], outputs=(0,)) as (a,):
+        seidel2d(a, TSTEPS)
 
 
 def channel_flow_dropin(nit, u, v, dt, dx, dy, p, rho, nu, F):

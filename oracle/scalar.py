@@ -377,3 +377,24 @@ def channel_flow(nit, u, v, dt, dx, dy, p, rho, nu, F):
         udiff = (su - sun) / su
         steps += 1
     return np.array(U), np.array(V), np.array(P), steps
+
+
+def seidel_2d(TSTEPS, N, A):
+    """seidel_2d_numpy.py:4-13, one point at a time: the 7-neighbour sum of a
+    row is formed from the row's values before its own sweep (NumPy builds
+    the whole right-hand side first).  Returns a new array."""
+    a = A.tolist()
+    for _ in range(TSTEPS - 1):
+        for i in range(1, N - 1):
+            up, mid, dn = a[i - 1], a[i][:], a[i + 1]
+            for j in range(1, N - 1):
+                s = up[j - 1] + up[j]
+                s = s + up[j + 1]
+                s = s + mid[j + 1]
+                s = s + dn[j - 1]
+                s = s + dn[j]
+                s = s + dn[j + 1]
+                a[i][j] = mid[j] + s
+            for j in range(1, N - 1):
+                a[i][j] = (a[i][j] + a[i][j - 1]) / 9.0
+    return np.array(a)

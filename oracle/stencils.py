@@ -14,6 +14,7 @@ Reference files (paths relative to the npbench source tree):
   fdtd_2d_kernel    npbench/benchmarks/polybench/fdtd_2d/fdtd_2d_numpy.py:4-11
   cavity_flow       npbench/benchmarks/cavity_flow/cavity_flow_numpy.py:13-83
   channel_flow      npbench/benchmarks/channel_flow/channel_flow_numpy.py:13-152
+  seidel_2d_kernel  npbench/benchmarks/polybench/seidel_2d/seidel_2d_numpy.py:4-13
 """
 
 import numpy as np
@@ -474,3 +475,20 @@ def channel_flow(nit, u, v, dt, dx, dy, p, rho, nu, F):
         udiff = (np.sum(u) - np.sum(un)) / np.sum(u)
         stepcount += 1
     return stepcount
+
+
+# --------------------------------------------------------------------------
+# Gauss-Seidel 2-D (polybench seidel_2d)
+# --------------------------------------------------------------------------
+def seidel_2d_kernel(TSTEPS, N, A):
+    """seidel_2d_numpy.kernel: TSTEPS - 1 sweeps in place; each row adds the
+    sum of its 7 neighbours (the row above already swept, the row below not)
+    and then runs A[i, j] = (A[i, j] + A[i, j-1]) / 9 along j."""
+    _f64(A)
+    for _ in range(0, TSTEPS - 1):
+        for i in range(1, N - 1):
+            A[i, 1:-1] += (A[i - 1, :-2] + A[i - 1, 1:-1] + A[i - 1, 2:] + A[i, 2:]
+                           + A[i + 1, :-2] + A[i + 1, 1:-1] + A[i + 1, 2:])
+            row = A[i]
+            for j in range(1, N - 1):
+                row[j] = (row[j] + row[j - 1]) / 9.0

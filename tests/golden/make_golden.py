@@ -25,6 +25,7 @@ Cases (all FP64):
               drives the flow); (5, 7) nt=3 nit=4 seeded u, v, p
   channel_flow (ny, nx) = (4, 5) nit=2 reference init (u = v = 0, p = 1;
               converges in 564 steps); (5, 6) nit=3 from rest with seeded v, p
+  seidel_2d   N=8 TSTEPS=4 reference init; N=7 TSTEPS=3 seeded uniform
 
 Arguments, if given, are name prefixes: only the matching files are written.
 """
@@ -48,6 +49,7 @@ from npbench_amd.benchmarks.polybench.jacobi_1d import jacobi_1d  # noqa: E402
 from npbench_amd.benchmarks.polybench.fdtd_2d import fdtd_2d  # noqa: E402
 from npbench_amd.benchmarks.cavity_flow import cavity_flow  # noqa: E402
 from npbench_amd.benchmarks.channel_flow import channel_flow  # noqa: E402
+from npbench_amd.benchmarks.polybench.seidel_2d import seidel_2d  # noqa: E402
 
 ONLY = tuple(sys.argv[1:])
 
@@ -140,6 +142,12 @@ def main():
         ou, ov, op, steps = scalar.channel_flow(nit, u, v, dt, dx, dy, p, 1.0, 0.1, 1.0)
         save(f"channel_flow_{tag}", nit=nit, dt=dt, dx=dx, dy=dy, rho=1.0, nu=0.1, F=1.0, u=u, v=v,
              p=p, u_out=ou, v_out=ov, p_out=op, stepcount=steps)
+
+    # Gauss-Seidel 2-D
+    rng = np.random.default_rng(29)
+    for tag, n, tsteps, A in (("ref", 8, 4, seidel_2d.initialize(8)),
+                              ("uniform", 7, 3, rng.random((7, 7)))):
+        save(f"seidel_2d_{tag}", N=n, TSTEPS=tsteps, A=A, out=scalar.seidel_2d(tsteps, n, A))
 
 
 if __name__ == "__main__":

@@ -80,6 +80,9 @@ def test_version_string():
                       5, 5, 10, None, None), "stepcount"),
     (lambda: nat.call("npbx_channel_flow", 0, 2, None, None, 0.1, 0.5, 0.5, None, 1.0, 0.1, 1.0,
                       5, 5, 10, ctypes.byref(ctypes.c_int64()), None), "distinct"),
+    (lambda: nat.call("npbx_seidel2d", 1, None, 10, 3, None), "dtype"),
+    (lambda: nat.call("npbx_seidel2d", 0, None, 5000, 3, None), "N=5000"),
+    (lambda: nat.call("npbx_seidel2d", 0, None, 10, 3, None), "null"),
 ])
 def test_argument_errors_raise_with_message(call, needle):
     with pytest.raises(nat.NpbxError) as ei:
@@ -101,6 +104,8 @@ def test_empty_work_is_a_no_op():
     assert nat.call("npbx_fdtd2d", 0, None, None, None, None, 4, 5, 0, None) == nat.OK
     assert nat.call("npbx_cavity_flow", 0, 5, 5, 0, 3, None, None, 0.1, 0.5, 0.5, None, 1.0, 0.1,
                     None) == nat.OK  # no time step
+    assert nat.call("npbx_seidel2d", 0, None, 2, 5, None) == nat.OK   # no interior
+    assert nat.call("npbx_seidel2d", 0, None, 10, 1, None) == nat.OK  # no sweep
 
 
 def test_last_error_buffer_is_terminated():

@@ -49,6 +49,12 @@ fails at least one check:
              step count (the periodic columns wrap, no column is special);
              brute force (to convergence, hundreds of steps) and golden
              vectors with their step counts.
+  seidel_2d  one sweep of a 4 x 4 grid holding 81 at (1, 1), worked by hand:
+             (1, 1) -> 9, (1, 2) -> 1 (its left neighbour already swept), row
+             2 sees the swept row 1 (sum 10): 10/9 and (10 + 10/9)/9 (pins
+             the sweep order, the seven-neighbour sum and the recurrence);
+             a constant field is a fixed point ((8c + c)/9 = c); brute force
+             and golden vectors.
 
 The last section plants mistakes in the oracle (monkeypatched copies) and
 checks the pins above catch them.
@@ -856,6 +862,62 @@ def test_planted_channel_mistakes_are_caught(monkeypatch):
                     test_channel_is_periodic_in_x()
 
 
+# ------------------------------------------------------------------ seidel 2-D
+def test_seidel2d_delta_by_hand():
+    A = np.zeros((4, 4))
+    A[1, 1] = 81.0
+    S.seidel_2d_kernel(2, 4, A)
+    want = np.zeros((4, 4))
+    want[1, 1], want[1, 2] = 9.0, 1.0
+    want[2, 1] = 10.0 / 9.0
+    want[2, 2] = (10.0 + 10.0 / 9.0) / 9.0
+    assert np.array_equal(A, want)
+
+
+def test_seidel2d_constant_is_fixed_and_boundary_untouched():
+    for c in (1.0, 0.5, 3.0):
+        A = np.full((9, 9), c)
+        S.seidel_2d_kernel(5, 9, A)
+        assert np.all(A == c)
+    rng = np.random.default_rng(2)
+    A = rng.random((8, 8))
+    B = A.copy()
+    S.seidel_2d_kernel(4, 8, B)
+    for edge in (np.s_[0, :], np.s_[-1, :], np.s_[:, 0], np.s_[:, -1]):
+        assert np.array_equal(A[edge], B[edge])
+
+
+@pytest.mark.parametrize("n,tsteps", [(3, 2), (4, 3), (5, 4), (9, 3), (12, 5)])
+def test_seidel2d_bruteforce_bitwise(n, tsteps):
+    A = np.random.default_rng(n * 7 + tsteps).random((n, n))
+    a = A.copy()
+    S.seidel_2d_kernel(tsteps, n, a)
+    assert np.array_equal(a, scalar.seidel_2d(tsteps, n, A))
+
+
+def test_planted_seidel2d_mistakes_are_caught(monkeypatch):
+    """A Jacobi-style sweep (old values of the row above), a divisor of 8
+    and a dropped neighbour each move the hand-worked delta response."""
+    def make(mistake):
+        def kernel(TSTEPS, N, A):
+            for _ in range(TSTEPS - 1):
+                old = A.copy()
+                for i in range(1, N - 1):
+                    up = old[i - 1] if mistake == "jacobi" else A[i - 1]
+                    s = up[:-2]
+                    if mistake != "drop":
+                        s = s + up[1:-1]
+                    s = s + up[2:] + A[i, 2:] + A[i + 1, :-2] + A[i + 1, 1:-1] + A[i + 1, 2:]
+                    A[i, 1:-1] += s
+                    for j in range(1, N - 1):
+                        A[i, j] = (A[i, j] + A[i, j - 1]) / (8.0 if mistake == "eight" else 9.0)
+        return kernel
+    for mistake in ("jacobi", "eight", "drop"):
+        monkeypatch.setattr(S, "seidel_2d_kernel", make(mistake))
+        with pytest.raises(AssertionError):
+            test_seidel2d_delta_by_hand()
+
+
 # ------------------------------------------------------------------ golden
 def _golden(name):
     with np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False) as z:
@@ -927,6 +989,14 @@ def test_golden_channel_flow(name):
     assert steps == int(g["stepcount"])
     assert np.array_equal(u, g["u_out"]) and np.array_equal(v, g["v_out"])
     assert np.array_equal(p, g["p_out"])
+
+
+@pytest.mark.parametrize("name", ["seidel_2d_ref", "seidel_2d_uniform"])
+def test_golden_seidel_2d(name):
+    g = _golden(name)
+    A = g["A"].copy()
+    S.seidel_2d_kernel(int(g["TSTEPS"]), int(g["N"]), A)
+    assert np.array_equal(A, g["out"])
 
 
 @pytest.mark.parametrize("name", ["adi_ref", "adi_uniform"])
