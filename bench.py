@@ -428,6 +428,25 @@ def kernel_suite(st, dev, torch, clock, reps, copy_only=False):
         "us_per_phase": round(ev / reps / (nsteps[-1] * (NITH + 3)) * 1e6, 3),
         "bound": "latency (one workgroup, a barrier per phase)"}
     del h0, h1
+
+    # Gauss-Seidel 2-D (polybench seidel_2d, FP64) at the paper preset: 99
+    # sweeps of 400 x 400 in wavefront slots (rows of successive sweeps two
+    # apart run together), each row's recurrence along j serial
+    from npbench_amd.benchmarks.polybench.seidel_2d import seidel_2d as sd_init
+    TS, NS = 100, 400
+    s0 = dev.to_device(sd_init.initialize(NS), torch.float64)
+    s1 = s0.clone()
+
+    def sd():
+        s1.copy_(s0)
+        st.seidel2d(s1, TS, stream=clock.stream)
+    sd()
+    _, ev = clock.run(sd, reps)
+    res[f"seidel_2d_paper_N{NS}_T{TS}_f64"] = {
+        "ms_per_call": round(ev / reps * 1e3, 4),
+        "glups": round((TS - 1) * (NS - 2) ** 2 * reps / ev / 1e9, 3),
+        "bound": "latency (a serial recurrence per row, rows in wavefront slots)"}
+    del s0, s1
     return res
 
 
@@ -496,6 +515,18 @@ def cpu_channel_ms():
     t0 = time.perf_counter()
     ref.channel_flow(50, u, v, dt, dx, dy, p, 1.0, 0.1, 1.0)
     return round((time.perf_counter() - t0) * 1e3, 1)
+
+
+def cpu_seidel_ms(sample_sweeps=10):
+    """The NumPy oracle's seidel_2d at the paper preset (N=400, 99 sweeps; 1
+    core) on a bounded sample of sample_sweeps sweeps, scaled to 99: every
+    sweep costs the same."""
+    from oracle import stencils as ref
+    from npbench_amd.benchmarks.polybench.seidel_2d import seidel_2d as sd_init
+    A = sd_init.initialize(400)
+    t0 = time.perf_counter()
+    ref.seidel_2d_kernel(sample_sweeps + 1, 400, A)
+    return round((time.perf_counter() - t0) * 1e3 * 99 / sample_sweeps, 1)
 
 
 def cpu_adi_ms():
@@ -643,6 +674,10 @@ def main():
             ch = line["kernels"]["channel_flow_paper_101x101_nit50_f64"]
             ch["cpu_oracle_ms"] = cpu_channel_ms()
             ch["speedup_vs_cpu_oracle"] = round(ch["cpu_oracle_ms"] / ch["ms_per_call"], 1)
+            sdl = line["kernels"]["seidel_2d_paper_N400_T100_f64"]
+            sdl["cpu_oracle_ms"] = cpu_seidel_ms()
+            sdl["cpu_oracle_sample"] = "10 of 99 sweeps, scaled by 9.9"
+            sdl["speedup_vs_cpu_oracle"] = round(sdl["cpu_oracle_ms"] / sdl["ms_per_call"], 1)
     if args.cpu_sweeps > 0:
         line["cpu_baseline"] = cpu_baseline(HEAT_N, args.cpu_sweeps)
     print(json.dumps(line), flush=True)
