@@ -265,9 +265,10 @@ int npbx_channel_flow(int dtype, int64_t nit, void *u, void *v, double dt,
 
 /* ---- 2-D Gauss-Seidel (seidel_2d_numpy.py:4-13) ----------------------------
  * A: (N, N) FP64 (dtype NPBX_F64), updated in place by TSTEPS - 1 sweeps;
- * bitwise the NumPy result.  N <= 1280 (a row per wave in LDS); N < 3 or
- * TSTEPS <= 1 is a no-op.  Rows run in wavefront slots (row i of sweep t
- * with row i+2 of sweep t-1), one workgroup, at most 64 sweeps a launch. */
+ * bitwise the NumPy result.  N <= 16384; N < 3 or TSTEPS <= 1 is a no-op.
+ * Rows run in wavefront slots (row i of sweep t with row i+2 of sweep t-1),
+ * one workgroup, at most 64 sweeps a launch; the rows of a slot run their
+ * recurrences side by side, one lane each. */
 int npbx_seidel2d(int dtype, void *A, int64_t N, int64_t tsteps, void *stream);
 
 /* ---- alternating-direction implicit solver (adi_numpy.py:6-52) -------------

@@ -8,12 +8,15 @@
 // recurrence.  Row i of sweep t needs row i-1 of sweep t and row i+1 of
 // sweep t-1, so every row with i + 2t = s can run in slot s once slot s-1
 // is done, and no two rows of a slot are neighbours.  One workgroup walks
-// the slots with a barrier between them; each wave takes rows of the slot:
-// its lanes form the neighbour sums (coalesced, in the reference's order)
-// into the wave's LDS row, then one lane runs the recurrence from LDS and
-// stores the row, with the short division by 9 (its reciprocal prepared
-// once) and the division operator for a row where the short form could not
-// vouch.  A launch runs at most kSweepsPerLaunch sweeps.
+// the slots with a barrier between them; a launch runs at most
+// kSweepsPerLaunch sweeps, so a slot holds at most that many rows, and their
+// recurrences run side by side in the lanes of one wave (the chain's cost is
+// paid once per slot, not once per row).  Per block of kBlockJ columns: the
+// workgroup forms every row's inputs (the neighbour sums, coalesced, in the
+// reference's order) into LDS, wave 0 runs the recurrences through the
+// block with the short division by 9 (its reciprocal prepared once; a lane
+// that cannot vouch for its block redoes it with the division operator), and
+// the workgroup stores the block, coalesced.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,84 +28,86 @@ namespace npbx {
 namespace {
 
 constexpr int kThreadsS = 1024;
-constexpr int kWavesS = kThreadsS / 64;
-constexpr int kMaxN = 1280;  // a row per wave in LDS: 16 x 1280 doubles = 160 KiB
-constexpr int kSweepsPerLaunch = 64;
-constexpr int kScanUnroll = 8;
+constexpr int kSweepsPerLaunch = 64;  // also the most rows a slot holds
+constexpr int kBlockJ = 256;          // columns per block: 256 x 64 doubles = 128 KiB of LDS
+constexpr int kMaxN = 1 << 14;
 
-// Orders the wave's LDS writes before its own lanes' reads.
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+// The reference's neighbour sum plus centre for row i at column j (:8-10).
+__device__ __forceinline__ double row_input(const double* up, const double* row,
+                                            const double* dn, int j) {
+  double s = up[j - 1] + up[j];
+  s = s + up[j + 1];
+  s = s + row[j + 1];
+  s = s + dn[j - 1];
+  s = s + dn[j];
+  s = s + dn[j + 1];
+  return row[j] + s;
 }
 
 __global__ __launch_bounds__(kThreadsS) void seidel2d_slots(double* __restrict__ A, int n,
                                                             int sweeps) {
-  __shared__ double rows[kWavesS][kMaxN];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  double* lb = rows[wave];
+  __shared__ double blk[kBlockJ * kSweepsPerLaunch];  // [column in block][row of the slot]
+  const int tid = threadIdx.x;
   double lo = 0.0, hi = INFINITY;
   const Divisor nine = prepare_divisor(9.0, lo, hi);
   const int last_slot = (n - 3) + 2 * (sweeps - 1);
   for (int slot = 0; slot <= last_slot; ++slot) {
-    // rows (t, i) with (i - 1) + 2 t = slot, t in [0, sweeps), i in [1, n-2]
+    // rows (t, i) with (i - 1) + 2 t = slot, t in [0, sweeps), i in [1, n-2];
+    // slot row r is t = t_hi - r
     const int t_lo = max(0, (slot - (n - 3) + 1) / 2);
     const int t_hi = min(sweeps - 1, slot / 2);
-    for (int t = t_hi - wave; t >= t_lo; t -= kWavesS) {
-      const int i = slot - 2 * t + 1;
-      const double* up = A + static_cast<int64_t>(i - 1) * n;
-      double* row = A + static_cast<int64_t>(i) * n;
-      const double* dn = A + static_cast<int64_t>(i + 1) * n;
-      // A[i, 1:-1] += (seven neighbours, left to right) (:8-10)
-      for (int j = 1 + lane; j < n - 1; j += 64) {
-        double s = up[j - 1] + up[j];
-        s = s + up[j + 1];
-        s = s + row[j + 1];
-        s = s + dn[j - 1];
-        s = s + dn[j];
-        s = s + dn[j + 1];
-        lb[j] = row[j] + s;
-      }
-      wave_lds_sync();
-      // A[i, j] = (A[i, j] + A[i, j-1]) / 9 for j = 1 .. n-2 (:11-13); the
-      // inputs stay in LDS, so a row the short form cannot vouch for is
-      // redone with the division operator
-      if (lane == 0) {
-        double x = row[0];
-        double mn = INFINITY, mx = 0.0;
-        int j = 1;
-        for (; j + kScanUnroll <= n - 1; j += kScanUnroll) {
-          double in[kScanUnroll];  // loaded ahead of the dependent chain
-#pragma unroll
-          for (int k = 0; k < kScanUnroll; ++k) in[k] = lb[j + k];
-#pragma unroll
-          for (int k = 0; k < kScanUnroll; ++k) {
-            double m;
-            x = short_division(in[k] + x, nine.d, nine.y, m);
-            mn = fmin(mn, fabs(m));
-            mx = fmax(mx, fabs(m));
-            row[j + k] = x;
-          }
+    const int count = t_hi - t_lo + 1;
+    double x = 0.0;  // lane r of wave 0: the recurrence's carry of slot row r
+    if (tid < count) x = A[static_cast<int64_t>(slot - 2 * (t_hi - tid) + 1) * n];
+    for (int jb = 1; jb < n - 1; jb += kBlockJ) {
+      const int bw = min(kBlockJ, n - 1 - jb);
+      // 1. inputs of every row of the slot for columns jb .. jb+bw-1, lanes
+      //    along j (coalesced), 4 rows at a time over the workgroup
+      for (int r = tid / kBlockJ; r < count; r += kThreadsS / kBlockJ) {
+        const int jj = tid % kBlockJ;
+        if (jj < bw) {
+          const int i = slot - 2 * (t_hi - r) + 1;
+          const double* row = A + static_cast<int64_t>(i) * n;
+          blk[jj * kSweepsPerLaunch + r] = row_input(row - n, row, row + n, jb + jj);
         }
-        for (; j < n - 1; ++j) {
+      }
+      __syncthreads();
+      // 2. the recurrences x_j = (in_j + x_{j-1}) / 9 (:11-13), one lane per
+      //    row, results over the inputs; a lane whose |m| left the exact range
+      //    recomputes its inputs (its row in memory still holds the old
+      //    values) and redoes the block with the division operator
+      if (tid < count) {
+        const int r = tid;
+        const double x0 = x;
+        double mn = INFINITY, mx = 0.0;
+        for (int jj = 0; jj < bw; ++jj) {
           double m;
-          x = short_division(lb[j] + x, nine.d, nine.y, m);
+          x = short_division(blk[jj * kSweepsPerLaunch + r] + x, nine.d, nine.y, m);
           mn = fmin(mn, fabs(m));
           mx = fmax(mx, fabs(m));
-          row[j] = x;
+          blk[jj * kSweepsPerLaunch + r] = x;
         }
         if (!(mn >= lo && mx < hi)) {
-          x = row[0];
-          for (int j = 1; j < n - 1; ++j) {
-            x = (lb[j] + x) / 9.0;
-            row[j] = x;
+          const int i = slot - 2 * (t_hi - r) + 1;
+          const double* row = A + static_cast<int64_t>(i) * n;
+          x = x0;
+          for (int jj = 0; jj < bw; ++jj) {
+            x = (row_input(row - n, row, row + n, jb + jj) + x) / 9.0;
+            blk[jj * kSweepsPerLaunch + r] = x;
           }
         }
       }
-      wave_lds_sync();  // the next row of this wave rewrites lb
+      __syncthreads();
+      // 3. the block's results back to the rows, coalesced
+      for (int r = tid / kBlockJ; r < count; r += kThreadsS / kBlockJ) {
+        const int jj = tid % kBlockJ;
+        if (jj < bw) {
+          const int i = slot - 2 * (t_hi - r) + 1;
+          A[static_cast<int64_t>(i) * n + jb + jj] = blk[jj * kSweepsPerLaunch + r];
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 

@@ -81,7 +81,7 @@ def test_version_string():
     (lambda: nat.call("npbx_channel_flow", 0, 2, None, None, 0.1, 0.5, 0.5, None, 1.0, 0.1, 1.0,
                       5, 5, 10, ctypes.byref(ctypes.c_int64()), None), "distinct"),
     (lambda: nat.call("npbx_seidel2d", 1, None, 10, 3, None), "dtype"),
-    (lambda: nat.call("npbx_seidel2d", 0, None, 5000, 3, None), "N=5000"),
+    (lambda: nat.call("npbx_seidel2d", 0, None, 20000, 3, None), "N=20000"),
     (lambda: nat.call("npbx_seidel2d", 0, None, 10, 3, None), "null"),
 ])
 def test_argument_errors_raise_with_message(call, needle):

@@ -3,8 +3,9 @@ oracle.
 
 Rows run in wavefront slots with the reference's arithmetic, so results must
 be bitwise the oracle's.  Sizes cover the smallest grid, rows shorter and
-longer than a wave, sweeps spanning two launches (at most 64 per launch), and
-the reference's initialisation at its S, M and L presets.
+longer than a 256-column block (one, two and three blocks), sweeps spanning
+two launches (at most 64 per launch), the division operator's path (zeros),
+and the reference's initialisation at its S, M and L presets.
 """
 
 import numpy as np
@@ -43,7 +44,7 @@ def check(st, tsteps, A, what):
     assert_bitwise(d.cpu().numpy(), want, what)
 
 
-@pytest.mark.parametrize("n", [3, 4, 5, 17, 64, 65, 66, 130])
+@pytest.mark.parametrize("n", [3, 4, 5, 17, 64, 65, 66, 130, 257, 258, 259, 600])
 @pytest.mark.parametrize("tsteps", [1, 2, 3, 7])
 def test_seidel2d_fp64_bitwise(st, n, tsteps):
     rng = np.random.default_rng(n * 100 + tsteps)
