@@ -48,7 +48,7 @@ typedef enum npbx_status {
   NPBX_EHIP = -2,         /* a HIP runtime call failed */
   NPBX_ETIMEDOUT = -3,    /* npbx_sync deadline passed; work still queued */
   NPBX_ENOMEM = -4,       /* device allocation failed */
-  NPBX_EUNSUPPORTED = -5  /* valid request this build does not implement */
+  NPBX_EUNSUPPORTED = -5  /* valid request the library does not implement */
 } npbx_status;
 
 typedef enum npbx_dtype {

@@ -8,7 +8,7 @@ every implementation the framework lists is timed ``repeat`` times after a
 first execution under the timeout; rows go to the ``results`` table of
 ``npbench.db`` (the reference's schema) and, with ``--jsonl``, to a JSON-lines
 file with GLUP/s.  The reference's line counting (LineCount, which needs
-pygount) and its DaCe SDFG options are not part of this build.
+pygount) and its DaCe SDFG options are not provided here.
 """
 
 import argparse
@@ -17,7 +17,7 @@ import sys
 from .infrastructure import Benchmark, Test, generate_framework
 from .infrastructure import utilities as util
 
-# The stencil path this build implements (bench_info/*.json).
+# The benchmarks with MI355X implementations (bench_info/*.json).
 BENCHMARKS = ("jacobi_2d", "heat_3d", "hdiff", "vadv", "adi", "jacobi_1d", "fdtd_2d",
               "cavity_flow", "channel_flow", "seidel_2d")
 
