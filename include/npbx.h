@@ -203,7 +203,9 @@ int npbx_pack_cols(int dtype, const void *A, const void *B, int64_t n,
  * 2: 32/8, 3: 8/4), "hdiff.tblock" = iterated hdiff steps fused per HBM round
  * trip (1, the default, 2 or 4; BF16 at most 2), "adi.path" = ADI launches (0,
  * the default: by size; 1: all time steps in one single-workgroup launch, up
- * to 512 lines; 2: one launch per half step).  Results do not depend on them. */
+ * to 512 lines; 2: one launch per half step), "seidel.blocks" = wavefront
+ * slot-blocks (slots x 256-column blocks) per seidel_2d launch (default 2048,
+ * about 40 ms).  Results do not depend on them. */
 int npbx_set_tuning(const char *key, int64_t value);
 int npbx_get_tuning(const char *key, int64_t *value);
 
@@ -267,8 +269,10 @@ int npbx_channel_flow(int dtype, int64_t nit, void *u, void *v, double dt,
  * A: (N, N) FP64 (dtype NPBX_F64), updated in place by TSTEPS - 1 sweeps;
  * bitwise the NumPy result.  N <= 16384; N < 3 or TSTEPS <= 1 is a no-op.
  * Rows run in wavefront slots (row i of sweep t with row i+2 of sweep t-1),
- * one workgroup, at most 64 sweeps a launch; the rows of a slot run their
- * recurrences side by side, one lane each. */
+ * one workgroup, at most 64 sweeps per slot schedule; the rows of a slot run
+ * their recurrences side by side, one lane each.  A launch runs a range of
+ * slots of at most "seidel.blocks" slot-blocks, so its duration is bounded
+ * whatever N is. */
 int npbx_seidel2d(int dtype, void *A, int64_t N, int64_t tsteps, void *stream);
 
 /* ---- alternating-direction implicit solver (adi_numpy.py:6-52) -------------

@@ -128,6 +128,10 @@ void set_hdiff_tblock(int tb);
 // single-workgroup launch, 2 = one launch per half step).
 int adi_path();
 void set_adi_path(int path);
+// Slot-blocks per seidel_2d launch ("seidel.blocks"; a slot-block is one
+// wavefront slot over one 256-column block).
+int seidel_blocks();
+void set_seidel_blocks(int blocks);
 
 // Per-device scratch owned by the library (grow-only, freed at exit).  Used
 // by kernels that need storage beyond their inputs and outputs.

@@ -8,10 +8,13 @@
 // recurrence.  Row i of sweep t needs row i-1 of sweep t and row i+1 of
 // sweep t-1, so every row with i + 2t = s can run in slot s once slot s-1
 // is done, and no two rows of a slot are neighbours.  One workgroup walks
-// the slots with a barrier between them; a launch runs at most
-// kSweepsPerLaunch sweeps, so a slot holds at most that many rows, and their
-// recurrences run side by side in the lanes of one wave (the chain's cost is
-// paid once per slot, not once per row).  Per block of kBlockJ columns: the
+// the slots with a barrier between them; a group of at most kSweepsPerLaunch
+// sweeps is one slot schedule, so a slot holds at most that many rows, and
+// their recurrences run side by side in the lanes of one wave (the chain's
+// cost is paid once per slot, not once per row).  A launch runs a range of
+// one group's slots holding a bounded number of slot-blocks (knob
+// "seidel.blocks"): all state is in A, so where the slots are split changes
+// nothing, and no launch runs longer than about that budget whatever N is.  Per block of kBlockJ columns: the
 // workgroup forms every row's inputs (the neighbour sums, coalesced, in the
 // reference's order) into LDS, wave 0 runs the recurrences through the
 // block with the short division by 9 (its reciprocal prepared once; a lane
@@ -20,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 
 #include "common.h"
 #include "division.h"
@@ -31,6 +35,10 @@ constexpr int kThreadsS = 1024;
 constexpr int kSweepsPerLaunch = 64;  // also the most rows a slot holds
 constexpr int kBlockJ = 256;          // columns per block: 256 x 64 doubles = 128 KiB of LDS
 constexpr int kMaxN = 1 << 14;
+
+// Slot-blocks (slots x 256-column blocks) per launch: about 18 us each for a
+// full slot, so the default keeps a launch under about 40 ms.
+std::atomic<int> g_seidel_blocks{2048};
 
 // The reference's neighbour sum plus centre for row i at column j (:8-10).
 __device__ __forceinline__ double row_input(const double* up, const double* row,
@@ -45,13 +53,13 @@ __device__ __forceinline__ double row_input(const double* up, const double* row,
 }
 
 __global__ __launch_bounds__(kThreadsS) void seidel2d_slots(double* __restrict__ A, int n,
-                                                            int sweeps) {
+                                                            int sweeps, int slot_begin,
+                                                            int slot_end) {
   __shared__ double blk[kBlockJ * kSweepsPerLaunch];  // [column in block][row of the slot]
   const int tid = threadIdx.x;
   double lo = 0.0, hi = INFINITY;
   const Divisor nine = prepare_divisor(9.0, lo, hi);
-  const int last_slot = (n - 3) + 2 * (sweeps - 1);
-  for (int slot = 0; slot <= last_slot; ++slot) {
+  for (int slot = slot_begin; slot < slot_end; ++slot) {
     // rows (t, i) with (i - 1) + 2 t = slot, t in [0, sweeps), i in [1, n-2];
     // slot row r is t = t_hi - r
     const int t_lo = max(0, (slot - (n - 3) + 1) / 2);
@@ -112,6 +120,10 @@ __global__ __launch_bounds__(kThreadsS) void seidel2d_slots(double* __restrict__
 }
 
 }  // namespace
+
+int seidel_blocks() { return g_seidel_blocks.load(); }
+void set_seidel_blocks(int blocks) { g_seidel_blocks.store(blocks); }
+
 }  // namespace npbx
 
 extern "C" int npbx_seidel2d(int dtype, void* A, int64_t N, int64_t tsteps, void* stream) {
@@ -124,11 +136,18 @@ extern "C" int npbx_seidel2d(int dtype, void* A, int64_t N, int64_t tsteps, void
   if (N < 3 || sweeps <= 0) return NPBX_OK;
   NPBX_REQUIRE(A != nullptr, "seidel2d: null array");
   const hipStream_t s = as_stream(stream);
+  const int n = static_cast<int>(N);
+  const int blocks_per_slot = static_cast<int>(cdiv(n - 2, kBlockJ));
+  const int slots_per_launch = std::max(1, g_seidel_blocks.load() / blocks_per_slot);
   for (int64_t done = 0; done < sweeps; done += kSweepsPerLaunch) {
     const int k = static_cast<int>(std::min<int64_t>(kSweepsPerLaunch, sweeps - done));
-    hipLaunchKernelGGL(seidel2d_slots, dim3(1), dim3(kThreadsS), 0, s, static_cast<double*>(A),
-                       static_cast<int>(N), k);
-    NPBX_LAUNCHED("seidel2d_slots");
+    const int slots = (n - 3) + 2 * (k - 1) + 1;  // slots 0 .. (n-3) + 2(k-1)
+    for (int s0 = 0; s0 < slots; s0 += slots_per_launch) {
+      const int s1 = std::min(slots, s0 + slots_per_launch);
+      hipLaunchKernelGGL(seidel2d_slots, dim3(1), dim3(kThreadsS), 0, s, static_cast<double*>(A),
+                         n, k, s0, s1);
+      NPBX_LAUNCHED("seidel2d_slots");
+    }
   }
   return NPBX_OK;
 }

@@ -1425,9 +1425,15 @@ extern "C" int npbx_set_tuning(const char* key, int64_t value) {
     set_adi_path(static_cast<int>(value));
     return NPBX_OK;
   }
+  if (std::strcmp(key, "seidel.blocks") == 0) {
+    NPBX_REQUIRE(value >= 1 && value <= (1 << 20),
+                 "set_tuning: seidel.blocks must be in [1, 1048576]");
+    set_seidel_blocks(static_cast<int>(value));
+    return NPBX_OK;
+  }
   return fail(NPBX_EINVAL,
               "set_tuning: unknown key '%s' (heat.rows, heat.seg, sweep.shape, hdiff.tblock, "
-              "adi.path)",
+              "adi.path, seidel.blocks)",
               key);
 }
 
@@ -1451,6 +1457,10 @@ extern "C" int npbx_get_tuning(const char* key, int64_t* value) {
   }
   if (std::strcmp(key, "hdiff.tblock") == 0) {
     *value = hdiff_tblock();
+    return NPBX_OK;
+  }
+  if (std::strcmp(key, "seidel.blocks") == 0) {
+    *value = seidel_blocks();
     return NPBX_OK;
   }
   return fail(NPBX_EINVAL, "get_tuning: unknown key '%s'", key);

@@ -83,6 +83,7 @@ def test_version_string():
     (lambda: nat.call("npbx_seidel2d", 1, None, 10, 3, None), "dtype"),
     (lambda: nat.call("npbx_seidel2d", 0, None, 20000, 3, None), "N=20000"),
     (lambda: nat.call("npbx_seidel2d", 0, None, 10, 3, None), "null"),
+    (lambda: nat.call("npbx_set_tuning", b"seidel.blocks", 0), "seidel.blocks"),
 ])
 def test_argument_errors_raise_with_message(call, needle):
     with pytest.raises(nat.NpbxError) as ei:

@@ -1142,3 +1142,37 @@ def test_planted_adi_mistakes_are_caught(monkeypatch, mistake):
     monkeypatch.setattr(S, "adi_column_sweep", mutant)
     with pytest.raises(AssertionError):
         test_adi_column_sweep_solves_its_systems(9)
+
+
+# ------------------------------------------------------------------ C oracle
+def test_c_seidel2d_delta_by_hand():
+    """The C restatement gives the hand-worked delta response."""
+    from oracle import c_oracle
+    A = np.zeros((4, 4))
+    A[1, 1] = 81.0
+    c_oracle.seidel_2d_kernel(2, 4, A)
+    want = np.zeros((4, 4))
+    want[1, 1], want[1, 2] = 9.0, 1.0
+    want[2, 1] = 10.0 / 9.0
+    want[2, 2] = (10.0 + 10.0 / 9.0) / 9.0
+    assert np.array_equal(A, want)
+
+
+@pytest.mark.parametrize("name", ["seidel_2d_ref", "seidel_2d_uniform"])
+def test_c_seidel2d_golden(name):
+    from oracle import c_oracle
+    g = _golden(name)
+    A = g["A"].copy()
+    c_oracle.seidel_2d_kernel(int(g["TSTEPS"]), int(g["N"]), A)
+    assert np.array_equal(A, g["out"])
+
+
+@pytest.mark.parametrize("n,tsteps", [(3, 1), (3, 2), (4, 3), (9, 5), (64, 4), (130, 3),
+                                      (257, 2)])
+def test_c_seidel2d_matches_numpy_oracle_bitwise(n, tsteps):
+    from oracle import c_oracle
+    A = np.random.default_rng(n + 1000 * tsteps).random((n, n))
+    a, b = A.copy(), A.copy()
+    S.seidel_2d_kernel(tsteps, n, a)
+    c_oracle.seidel_2d_kernel(tsteps, n, b)
+    assert np.array_equal(a, b)
