@@ -282,6 +282,19 @@ int npbx_seidel2d(int dtype, void *A, int64_t N, int64_t tsteps, void *stream);
  * unchanged; TSTEPS == 0 is an error (the reference divides by it). */
 int npbx_adi(int dtype, void *u, int64_t N, int64_t tsteps, void *stream);
 
+/* npbx_adi without allocation.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, kernels and
+ * results as npbx_adi (bitwise), but its scratch is a fixed 16 MiB array
+ * inside the library, one per GPU, held until the process ends, instead of
+ * the grow-only library workspace: N up to 1000 fits, and a larger N is
+ * rejected with NPBX_EINVAL before anything is enqueued.  Calls no hipMalloc
+ * or hipFree.  Every call on one GPU shares that GPU's array, so calls that
+ * overlap on one GPU (different streams, or threads) race: serialise them.
+ * The array's device address is resolved once per device, by npbx_init or
+ * else by the first call on that device. */
+int npbx_adi_noalloc(int dtype, void *u, int64_t N, int64_t tsteps, void *stream);
+
 /* Self-check of npbx_adi's short division (no counterpart in the reference;
  * for tests): `pairs` seeded random FP64 quotients, half of them next to a
  * rounding midpoint, through the shortened IEEE division sequence the ADI

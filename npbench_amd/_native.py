@@ -91,6 +91,7 @@ SIGNATURES = {
     "npbx_hdiff": [_int, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _vp],
     "npbx_vadv": [_int, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _i64, _i64, _i64, _vp],
     "npbx_adi": [_int, _vp, _i64, _i64, _vp],
+    "npbx_adi_noalloc": [_int, _vp, _i64, _i64, _vp],
     "npbx_jacobi1d": [_int, _vp, _vp, _i64, _i64, _vp],
     "npbx_fdtd2d": [_int, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp],
     "npbx_cavity_flow": [_int, _i64, _i64, _i64, _i64, _vp, _vp, ctypes.c_double, ctypes.c_double,

@@ -39,6 +39,7 @@
 
 #include <atomic>
 #include <cstdint>
+#include <mutex>
 #include <type_traits>
 
 #include "common.h"
@@ -572,25 +573,44 @@ int launch_halves(const AdiArgs& g, int nb, int64_t tsteps, hipStream_t stream) 
   return NPBX_OK;
 }
 
-int run_adi(double* u, int64_t n64, int64_t tsteps, hipStream_t stream) {
+// Launch shape and scratch layout of one call: v (the column sweep's output,
+// (n + 16) rows of ldv), q ([step][line], qrows x ld) and the eight tables.
+struct AdiPlan {
+  int path, nt, nb;
+  int64_t ld, ldv, qrows, tl, vel, qel, tel;
+  size_t bytes() const { return static_cast<size_t>(vel + qel + tel) * sizeof(double); }
+};
+
+int plan_adi(int64_t n64, AdiPlan& pl) {
   const int n = static_cast<int>(n64);
   const int lines = n - 2;
-  int path = g_adi_path.load();
-  if (path == 0) path = 3;
-  NPBX_REQUIRE(path != 1 || lines <= 512,
+  pl.path = g_adi_path.load();
+  if (pl.path == 0) pl.path = 3;
+  NPBX_REQUIRE(pl.path != 1 || lines <= 512,
                "adi: one-launch path holds at most 512 lines (N=%d)", n);
-  const int nt = path == 1 ? (lines <= 256 ? 256 : 512) : path == 3 ? 64 : 256;
-  const int nb = static_cast<int>(cdiv(lines, nt));
-  const int64_t ld = static_cast<int64_t>(nb) * nt;
-  const int64_t ldv = ld + 1;
-  const int64_t qrows = cdiv(lines, kCW) * kCW + 1;
-  const int64_t tl = qrows + 1;  // table length: every chunk's block
-  const int64_t vel = (n64 + 16) * ldv, qel = qrows * ld, tel = 8 * tl;
-  void* ws = nullptr;
-  NPBX_TRY(workspace(static_cast<size_t>(vel + qel + tel) * sizeof(double), &ws));
+  pl.nt = pl.path == 1 ? (lines <= 256 ? 256 : 512) : pl.path == 3 ? 64 : 256;
+  pl.nb = static_cast<int>(cdiv(lines, pl.nt));
+  pl.ld = static_cast<int64_t>(pl.nb) * pl.nt;
+  pl.ldv = pl.ld + 1;
+  pl.qrows = cdiv(lines, kCW) * kCW + 1;
+  pl.tl = pl.qrows + 1;  // table length: every chunk's block
+  pl.vel = (n64 + 16) * pl.ldv;
+  pl.qel = pl.qrows * pl.ld;
+  pl.tel = 8 * pl.tl;
+  return NPBX_OK;
+}
+
+// Runs the call with ws, pl.bytes() of device memory; what ws holds on entry
+// does not reach u (the library workspace it normally is keeps whatever the
+// previous call left).
+int run_adi(double* u, int64_t n64, int64_t tsteps, const AdiPlan& pl, void* ws,
+            hipStream_t stream) {
+  const int n = static_cast<int>(n64);
+  const int path = pl.path, nt = pl.nt, nb = pl.nb;
+  const int64_t ld = pl.ld, ldv = pl.ldv, qrows = pl.qrows, tl = pl.tl;
   double* v = static_cast<double*>(ws);
-  double* qs = v + vel;
-  double* tab = qs + qel;
+  double* qs = v + pl.vel;
+  double* tab = qs + pl.qel;
   const AdiCoef k = adi_coef(tsteps, n64);
   hipLaunchKernelGGL(adi_tables, dim3(1), dim3(64), 0, stream, k, n, static_cast<int>(tl), tab);
   NPBX_LAUNCHED("adi_tables");
@@ -640,7 +660,79 @@ extern "C" int npbx_adi(int dtype, void* u, int64_t N, int64_t tsteps, void* str
   NPBX_REQUIRE(N <= 32768 && tsteps <= (int64_t(1) << 30), "adi: N or TSTEPS out of range");
   if (N < 3 || tsteps < 0) return NPBX_OK;  // no interior line, or no time step
   NPBX_REQUIRE(u, "adi: null array");
-  return run_adi(static_cast<double*>(u), N, tsteps, as_stream(stream));
+  AdiPlan pl;
+  NPBX_TRY(plan_adi(N, pl));
+  void* ws = nullptr;
+  NPBX_TRY(workspace(pl.bytes(), &ws));
+  return run_adi(static_cast<double*>(u), N, tsteps, pl, ws, as_stream(stream));
+}
+
+// ---- allocation-free entry point ---------------------------------------------
+// npbx_adi_noalloc runs npbx_adi's kernels with their scratch in this
+// file-scope device array instead of the library workspace, so it never
+// allocates.  Every GPU the library runs it on holds its own copy (16 MiB)
+// until the process ends.  In host code `scratch` names a host-side stand-in,
+// not device memory: kernels get the device address resolved per device by
+// hipGetSymbolAddress (adi_scratch_ptr), never the array's name.
+namespace npbx {
+namespace {
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;
+// The largest N accepted: its scratch (v, q and the tables, AdiPlan) is
+// 16,661,568 bytes on every launch path that takes N=1000.
+constexpr int64_t kNoAllocMaxN = 1000;
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The current device's address of `scratch`, resolved once per device with
+// that device current.
+int adi_scratch_ptr(void** out) {
+  int dev = 0;
+  NPBX_HIP(hipGetDevice(&dev));
+  NPBX_REQUIRE(dev >= 0 && dev < kMaxScratchDevices, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    NPBX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    NPBX_REQUIRE(p != nullptr, "adi_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return NPBX_OK;
+}
+}  // namespace
+
+int adi_noalloc_setup() {
+  void* p = nullptr;
+  return adi_scratch_ptr(&p);
+}
+}  // namespace npbx
+
+extern "C" int npbx_adi_noalloc(int dtype, void* u, int64_t N, int64_t tsteps, void* stream) {
+  using namespace npbx;
+  NPBX_REQUIRE(dtype == NPBX_F64, "adi_noalloc: dtype %s not supported (f64)",
+               dtype_name(dtype));
+  NPBX_REQUIRE(tsteps != 0, "adi_noalloc: TSTEPS=0 (the reference divides by it)");
+  NPBX_REQUIRE(N > 0, "adi_noalloc: N=%lld must be positive", static_cast<long long>(N));
+  NPBX_REQUIRE(N <= 32768 && tsteps <= (int64_t(1) << 30),
+               "adi_noalloc: N or TSTEPS out of range");
+  if (N < 3 || tsteps < 0) return NPBX_OK;
+  NPBX_REQUIRE(u, "adi_noalloc: null array");
+  NPBX_REQUIRE(N <= kNoAllocMaxN,
+               "adi_noalloc: N=%lld exceeds %lld, the most its fixed %zu-byte scratch holds; "
+               "use npbx_adi",
+               static_cast<long long>(N), static_cast<long long>(kNoAllocMaxN),
+               kNoAllocScratchBytes);
+  AdiPlan pl;
+  NPBX_TRY(plan_adi(N, pl));
+  NPBX_REQUIRE(pl.bytes() <= kNoAllocScratchBytes,
+               "adi_noalloc: N=%lld needs %zu bytes of scratch, more than its fixed %zu",
+               static_cast<long long>(N), pl.bytes(), kNoAllocScratchBytes);
+  void* ws = nullptr;
+  NPBX_TRY(adi_scratch_ptr(&ws));
+  return run_adi(static_cast<double*>(u), N, tsteps, pl, ws, as_stream(stream));
 }
 
 extern "C" int npbx_adi_division_check(int64_t pairs, uint64_t seed, int64_t* checked,

@@ -133,6 +133,10 @@ void set_adi_path(int path);
 int seidel_blocks();
 void set_seidel_blocks(int blocks);
 
+// Resolves the current device's address of npbx_adi_noalloc's fixed scratch
+// array (once per device; npbx_init calls it).
+int adi_noalloc_setup();
+
 // Per-device scratch owned by the library (grow-only, freed at exit).  Used
 // by kernels that need storage beyond their inputs and outputs.
 int workspace(size_t bytes, void** ptr);

@@ -130,7 +130,9 @@ int npbx_init(int device) {
   if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     return fail(NPBX_EUNSUPPORTED, "device %d is %s; libnpbx is built for gfx950 only", device,
                 prop.gcnArchName);
-  return NPBX_OK;
+  // Set-up of the allocation-free entry point: its fixed scratch array's
+  // address on this device.
+  return npbx::adi_noalloc_setup();
 }
 
 // Releases the library-owned workspace of the current device (after the

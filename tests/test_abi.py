@@ -84,6 +84,10 @@ def test_version_string():
     (lambda: nat.call("npbx_seidel2d", 0, None, 20000, 3, None), "N=20000"),
     (lambda: nat.call("npbx_seidel2d", 0, None, 10, 3, None), "null"),
     (lambda: nat.call("npbx_set_tuning", b"seidel.blocks", 0), "seidel.blocks"),
+    (lambda: nat.call("npbx_adi_noalloc", 0, None, 10, 5, None), "null"),
+    (lambda: nat.call("npbx_adi_noalloc", 1, None, 10, 5, None), "dtype"),
+    # rejected by the size check before any HIP call (the pointer is never used)
+    (lambda: nat.call("npbx_adi_noalloc", 0, ctypes.c_void_p(256), 1001, 3, None), "N=1001"),
 ])
 def test_argument_errors_raise_with_message(call, needle):
     with pytest.raises(nat.NpbxError) as ei:
