@@ -21,6 +21,7 @@ launched on.  Prints ONE JSON line.
 """
 
 import argparse
+import hashlib
 import json
 import os
 import platform
@@ -44,6 +45,7 @@ SIZE = {"f64": 8, "f32": 4, "bf16": 2}
 SERVE_HEAT_ROWS = 10  # serving's heat tiles: the wave-private FP32 tile (10 rows
                       # measured lowest p50/p99 on the serving mix)
 SERVE_TBLOCK = 2
+WARM_S = 0.1  # untimed calls before each per-kernel timing
 SERVE_SEG = 32  # heat planes per tile for serving (set explicitly: the process-wide
                 # knob holds whatever the headline's tuner chose)
 
@@ -66,8 +68,18 @@ class Clock:
         self.stream = stream
         self.timer = dev.Timer(stream)
 
-    def run(self, fn, reps):
+    def run(self, fn, reps, warm_s=0.0):
+        """Times reps calls of fn; with warm_s > 0, first keeps calling fn
+        (untimed) for about warm_s seconds so that the clocks of a GPU that
+        sat idle (while the host ran the oracle) are back up."""
         import torch
+        if warm_s > 0:
+            t_end = time.perf_counter() + warm_s
+            while True:
+                fn()
+                torch.cuda.synchronize()
+                if time.perf_counter() >= t_end:
+                    break
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         self.timer.start()
@@ -210,7 +222,40 @@ def serving(torch, args, heat_rows=SERVE_HEAT_ROWS, cpu_rates=None, tblock=SERVE
     srv.close()
     if cpu_rates:
         out["cpu_reference"] = cpu_serving_reference(reqs, cpu_rates)
+        out["cpu_reference"]["measured_sample"] = cpu_serving_sample(reqs, cpu_rates, torch)
     return out
+
+
+def cpu_serving_sample(reqs, rates, torch, budget_s=12.0, max_one_s=2.5):
+    """Measured, not modelled: requests of the mix drawn at random (seed 5)
+    whose modelled service time is at most max_one_s, each served by the
+    NumPy FP64 oracle on one host core from its own seeded inputs, until
+    about budget_s of CPU time is spent.  Reports every sampled request's
+    measured and modelled service time and their median ratio, which checks
+    the model the full-mix figures above rest on."""
+    from npbench_amd import serve
+    from oracle import stencils as ref
+    order = np.random.default_rng(5).permutation(len(reqs))
+    spent, rows = 0.0, []
+    for i in order:
+        r = reqs[int(i)]
+        model_s = r.lups / (rates[r.kind] * 1e9)
+        if model_s > max_one_s or spent + model_s > budget_s:
+            continue
+        A, B = serve.make_inputs(r, torch.float64)
+        a, b = A.cpu().numpy(), B.cpu().numpy()
+        f = ref.jacobi_sweeps if r.kind == "jacobi_2d" else ref.heat_sweeps
+        t0 = time.perf_counter()
+        f(a, b, r.sweeps)
+        t = time.perf_counter() - t0
+        spent += t
+        rows.append({"rid": r.rid, "kind": r.kind, "n": r.n, "steps": r.steps,
+                     "measured_ms": round(t * 1e3, 2), "modelled_ms": round(model_s * 1e3, 2)})
+    ratios = [x["measured_ms"] / x["modelled_ms"] for x in rows if x["modelled_ms"] > 0]
+    return {"requests": rows, "cores": 1,
+            "median_measured_over_modelled": round(float(np.median(ratios)), 3) if ratios else None,
+            "note": f"random requests with a modelled service time <= {max_one_s} s, about "
+                    f"{budget_s} s of oracle time in all"}
 
 
 def cpu_serving_reference(reqs, rates):
@@ -236,60 +281,148 @@ def cpu_serving_reference(reqs, rates):
                     "host); latencies from a single-server FCFS queue at the same arrivals"}
 
 
-# ----------------------------------------------------------------- kernels
-def kernel_suite(st, dev, torch, clock, reps, copy_only=False):
-    res = {}
-    rng = np.random.default_rng(42)
+# ----------------------------------------------------------------- configurations
+def input_hash(*arrays):
+    """sha256 (first 16 hex digits) of the FP64 inputs' bytes: identifies the
+    seeded inputs a figure was measured on (numpy's Generator streams are not
+    guaranteed across versions)."""
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+    return h.hexdigest()[:16]
+
+
+def error_vs(x, ref):
+    """inf-norm relative error max|x - ref| / max|ref|, the elementwise
+    maximum of |x - ref| / max(|ref|, 1e-3 max|ref|), and the reference
+    harness's 2-norm ratio (utilities.py:27-28)."""
+    d = np.abs(x - ref)
+    scale = float(np.max(np.abs(ref)))
+    return {"inf_norm": float(d.max()) / scale,
+            "elementwise": float(np.max(d / np.maximum(np.abs(ref), 1e-3 * scale))),
+            "norm2": float(np.linalg.norm(d) / np.linalg.norm(ref))}
+
+
+def oracle_timed(fn, inputs, repeat, outputs):
+    """The NumPy oracle with the harness's timing (utilities.py:135-151):
+    timeit.repeat(number=1) with fresh copies of the inputs made in the
+    untimed setup, median reported.  Returns (median seconds, the outputs of
+    the last repetition)."""
+    from npbench_amd.infrastructure import utilities as util
+    box = [None]
+    ctx = {"__f": fn, "__in": inputs, "__box": box, "np": np}
+    _, times = util.benchmark(
+        "__f(*__box[0])",
+        setup="__box[0] = [x.copy() if isinstance(x, np.ndarray) else x for x in __in]",
+        repeat=repeat, context=ctx, verbose=False)
+    return statistics.median(times), [box[0][i] for i in outputs]
+
+
+def quantized(a, torch, dt):
+    return torch.from_numpy(a).to(dt).double().numpy()
+
+
+def config_suite(st, dev, torch, clock, reps, cpu):
+    """The measured configurations on their own seeded inputs (numpy
+    default_rng(42)): 2-D Jacobi 1000^2 with 200 sweeps, hdiff and vadv
+    256 x 256 x 160.  Per storage form: GPU time (HIP events on the launch
+    stream), the fraction of 8 TB/s of the algorithmic bytes, and, with
+    ``cpu``, the error against the FP64 oracle (FP64: bitwise or not) and the
+    oracle's own time on this host with the harness's semantics (1 core)."""
     from npbench_amd import _native as nat
+    from oracle import stencils as ref
+    res = {}
+    dts = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16}
+    host = cpu_model()
 
-    nbytes = 1 << 31
-    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
-    dst = torch.empty_like(src)
-
-    def cp():
-        nat.call("npbx_stream_copy", dst.data_ptr(), src.data_ptr(), nbytes,
-                 dev.stream_handle(clock.stream))
-    cp()
-    _, ev = clock.run(cp, reps)
-    res["stream_copy_2GiB"] = {"GBps": round(2 * nbytes / (ev / reps) / 1e9, 1),
-                               "bytes_per_launch": 2 * nbytes}
-    del src, dst
-    if copy_only:
-        return res
-
+    # --- 2-D Jacobi, 1000 x 1000, TSTEPS=101 (200 sweeps)
+    from npbench_amd.benchmarks.polybench.jacobi_2d.jacobi_2d import initialize_uniform
     n = 1000
-    A = dev.to_device(rng.random((n, n)))
-    B = dev.to_device(rng.random((n, n)))
-
-    def jac():
-        st.jacobi2d_sweeps(A, B, SWEEPS, stream=clock.stream)
-    jac()
-    wall, ev = clock.run(jac, reps)
+    A, B = initialize_uniform(n, seed=42)
     lups = (n - 2) ** 2 * SWEEPS
-    res["jacobi2d_1000_f64"] = {"glups": round(lups * reps / wall / 1e9, 2),
-                                "avg_sweep_us": round(ev / reps / SWEEPS * 1e6, 3),
-                                "hbm_frac": round(16 * lups * reps / ev / 1e9 / HBM_PEAK_GBS, 4)}
-    del A, B
+    want = None
+    if cpu:
+        t, want = oracle_timed(ref.jacobi_2d_kernel, [TSTEPS, A, B], 3, (1, 2))
+        res["jacobi_2d_1000_cpu_oracle"] = {
+            "ms_per_call": round(t * 1e3, 1), "glups": round(lups / t / 1e9, 4), "repeats": 3,
+            "cores": 1, "host": host, "timing": "median of timeit.repeat(number=1), copies "
+                                                 "untimed (the harness's)"}
+    for tag in ("f64", "f32", "bf16"):
+        a, b = dev.to_device(A, dts[tag]), dev.to_device(B, dts[tag])
 
+        def jac():
+            st.jacobi2d_sweeps(a, b, SWEEPS, stream=clock.stream)
+        jac()  # from the seeded inputs: the result checked below
+        line = {}
+        if want is not None:
+            torch.cuda.synchronize()
+            got = [a.double().cpu().numpy(), b.double().cpu().numpy()]
+            if tag == "f64":
+                line["bitwise_vs_oracle"] = bool(all(np.array_equal(g, w)
+                                                     for g, w in zip(got, want)))
+            else:
+                e = [error_vs(g, w) for g, w in zip(got, want)]
+                line["max_rel_err_vs_fp64_oracle"] = max(x["inf_norm"] for x in e)
+                line["error_vs_fp64_oracle"] = {k: max(x[k] for x in e) for k in e[0]}
+        wall, ev = clock.run(jac, reps, WARM_S)
+        line.update({"glups": round(lups * reps / ev / 1e9, 2),
+                     "avg_sweep_us": round(ev / reps / SWEEPS * 1e6, 3),
+                     "hbm_frac": round(2 * SIZE[tag] * lups * reps / ev / 1e9 / HBM_PEAK_GBS, 4),
+                     "input_sha256_16": input_hash(A, B)})
+        if cpu:
+            line["speedup_vs_cpu_oracle"] = round(
+                res["jacobi_2d_1000_cpu_oracle"]["ms_per_call"] / (ev / reps * 1e3), 1)
+        res[f"jacobi2d_1000_{tag}"] = line
+        del a, b
+
+    # --- hdiff, 256 x 256 x 160, coeff = 0.25 U
+    from npbench_amd.benchmarks.weather_stencils.hdiff.hdiff import initialize_config as hd_init
     I, J, K = 256, 256, 160
-    inf = rng.random((I + 4, J + 4, K))
-    coeff = 0.25 * rng.random((I, J, K))
-    for tag, dt in (("f64", torch.float64), ("f32", torch.float32), ("bf16", torch.bfloat16)):
+    inf, out0, coeff = hd_init(I, J, K, seed=42)
+    want = None
+    if cpu:
+        t, (want,) = oracle_timed(ref.hdiff, [inf, out0, coeff], 5, (1,))
+        res["hdiff_256x256x160_cpu_oracle"] = {
+            "ms_per_call": round(t * 1e3, 1), "glups": round(I * J * K / t / 1e9, 4),
+            "repeats": 5, "cores": 1, "host": host}
+    for tag in ("f64", "f32", "bf16"):
+        dt = dts[tag]
         di, dc = dev.to_device(inf, dt), dev.to_device(coeff, dt)
         do = torch.empty((I, J, K), dtype=dt, device="cuda")
 
         def hd():
             st.hdiff(di, do, dc, stream=clock.stream)
         hd()
-        _, ev = clock.run(hd, reps)
+        line = {}
+        if want is not None:
+            torch.cuda.synchronize()
+            got = do.double().cpu().numpy()
+            if tag == "f64":
+                line["bitwise_vs_oracle"] = bool(np.array_equal(got, want))
+            else:
+                e = error_vs(got, want)
+                line["max_rel_err_vs_fp64_oracle"] = e["inf_norm"]
+                line["error_vs_fp64_oracle"] = e
+                line["share_beyond_1e-5"] = float(
+                    np.mean(np.abs(got - want) > 1e-5 * np.max(np.abs(want))))
+                rq = out0.copy()
+                ref.hdiff(quantized(inf, torch, dt), rq, quantized(coeff, torch, dt))
+                line["max_rel_err_vs_same_input_oracle"] = error_vs(got, rq)["inf_norm"]
+        _, ev = clock.run(hd, reps, WARM_S)
         byts = SIZE[tag] * ((I + 4) * (J + 4) * K + 2 * I * J * K)
-        res[f"hdiff_256x256x160_{tag}"] = {
-            "glups": round(I * J * K * reps / ev / 1e9, 2), "avg_us": round(ev / reps * 1e6, 3),
-            "hbm_frac": round(byts * reps / ev / 1e9 / HBM_PEAK_GBS, 4)}
+        line.update({"glups": round(I * J * K * reps / ev / 1e9, 2),
+                     "avg_us": round(ev / reps * 1e6, 3),
+                     "hbm_frac": round(byts * reps / ev / 1e9 / HBM_PEAK_GBS, 4),
+                     "input_sha256_16": input_hash(inf, coeff)})
+        if cpu:
+            line["speedup_vs_cpu_oracle"] = round(
+                res["hdiff_256x256x160_cpu_oracle"]["ms_per_call"] / (ev / reps * 1e3), 1)
+        res[f"hdiff_256x256x160_{tag}"] = line
         if tag in ("f32", "bf16"):
             # iterated form: one launch per step (tblock 1) or tblock steps
-            # fused per HBM round trip; hbm_frac counts one step's bytes per
-            # step, so it exceeds the roofline's meaning once steps are fused
+            # fused per HBM round trip; effective_frac counts one step's
+            # bytes per step, so it exceeds the roofline's meaning once steps
+            # are fused
             tbs = (1, 2, 4) if tag == "f32" else (1, 2)
             for steps in (4, 8):
                 for tb in tbs:
@@ -298,7 +431,7 @@ def kernel_suite(st, dev, torch, clock, reps, copy_only=False):
                     def hdn(steps=steps):
                         st.hdiff(di, do, dc, nsteps=steps, stream=clock.stream)
                     hdn()
-                    _, ev = clock.run(hdn, reps)
+                    _, ev = clock.run(hdn, reps, WARM_S)
                     res[f"hdiff_256x256x160_{tag}_steps{steps}_tblock{tb}"] = {
                         "glups": round(I * J * K * steps * reps / ev / 1e9, 2),
                         "avg_step_us": round(ev / reps / steps * 1e6, 3),
@@ -306,21 +439,96 @@ def kernel_suite(st, dev, torch, clock, reps, copy_only=False):
             nat.set_tuning("hdiff.tblock", 1)
         del di, dc, do
 
-    fields = [rng.random((I, J, K)), rng.random((I, J, K)), 2 * rng.random((I + 1, J, K)) - 1,
-              rng.random((I, J, K)), rng.random((I, J, K))]
-    for tag, dt in (("f64", torch.float64), ("f32", torch.float32)):
-        d = [dev.to_device(a, dt) for a in fields]
+    # --- vadv, 256 x 256 x 160, wcon = 2U - 1, dtr_stage = 0.15
+    from npbench_amd.benchmarks.weather_stencils.vadv.vadv import initialize_config as va_init
+    dtr, *fields = va_init(I, J, K, seed=42)
+    want = None
+    if cpu:
+        t, (want,) = oracle_timed(ref.vadv, [*fields, dtr], 3, (0,))
+        res["vadv_256x256x160_cpu_oracle"] = {
+            "ms_per_call": round(t * 1e3, 1), "glups": round(I * J * K / t / 1e9, 4),
+            "repeats": 3, "cores": 1, "host": host}
+    for tag in ("f64", "f32"):
+        d0 = [dev.to_device(a, dts[tag]) for a in fields]
+        d = [x.clone() for x in d0]
 
         def va():
-            st.vadv(*d, 0.15, stream=clock.stream)
+            st.vadv(*d, dtr, stream=clock.stream)
         va()
-        _, ev = clock.run(va, reps)
+        line = {}
+        if want is not None:
+            torch.cuda.synchronize()
+            got = d[0].double().cpu().numpy()
+            if tag == "f64":
+                line["bitwise_vs_oracle"] = bool(np.array_equal(got, want))
+            else:
+                e = error_vs(got, want)
+                line["max_rel_err_vs_fp64_oracle"] = e["inf_norm"]
+                line["error_vs_fp64_oracle"] = e
+                q = [quantized(a, torch, dts[tag]) for a in fields]
+                ref.vadv(q[0], *q[1:], dtr)
+                line["max_rel_err_vs_same_input_oracle"] = error_vs(got, q[0])["inf_norm"]
+                line["note"] = ("wcon in [-1, 1) has near-zero pivots that amplify the FP32 "
+                                "rounding of the inputs; the oracle fed the same FP32 inputs "
+                                "differs from the FP64 oracle by the same amount")
+        _, ev = clock.run(va, reps, WARM_S)
         byts = SIZE[tag] * (5 * I * J * K + J * K + I * J * K)
-        res[f"vadv_256x256x160_{tag}"] = {
-            "glups": round(I * J * K * reps / ev / 1e9, 2), "avg_us": round(ev / reps * 1e6, 3),
-            "hbm_frac": round(byts * reps / ev / 1e9 / HBM_PEAK_GBS, 4)}
-        del d
+        line.update({"glups": round(I * J * K * reps / ev / 1e9, 2),
+                     "avg_us": round(ev / reps * 1e6, 3),
+                     "hbm_frac": round(byts * reps / ev / 1e9 / HBM_PEAK_GBS, 4),
+                     "input_sha256_16": input_hash(*fields)})
+        if cpu:
+            line["speedup_vs_cpu_oracle"] = round(
+                res["vadv_256x256x160_cpu_oracle"]["ms_per_call"] / (ev / reps * 1e3), 1)
+        res[f"vadv_256x256x160_{tag}"] = line
+        del d0, d
+    return res
 
+
+# ----------------------------------------------------------------- kernels
+BW_VARIANTS = {0: "U4", 1: "U8", 2: "U4 nontemporal", 3: "U8 nontemporal",
+               4: "U16 nontemporal"}
+
+
+def bandwidth(dev, torch, clock, reps, nbytes=1 << 31):
+    """The device's practical HBM ceiling: copy (2 GiB read + 2 GiB written),
+    read-only and write-only probes (2 GiB each) in every launch shape of
+    npbx_bandwidth plus the grid-stride npbx_stream_copy; GB/s from HIP
+    events, bytes moved = bytes read + bytes written.  The best of each kind
+    is the ceiling the stencil figures are held to."""
+    from npbench_amd import _native as nat
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    src.uniform_()
+    dst = torch.empty_like(src)
+    h = dev.stream_handle(clock.stream)
+    out = {}
+
+    def cp():
+        nat.call("npbx_stream_copy", dst.data_ptr(), src.data_ptr(), nbytes, h)
+    _, ev = clock.run(cp, reps, WARM_S)
+    out["copy grid-stride"] = round(2 * nbytes / (ev / reps) / 1e9, 1)
+    for mode, name, moved in ((0, "copy", 2), (1, "read", 1), (2, "write", 1)):
+        for v, vname in BW_VARIANTS.items():
+            def fn(mode=mode, v=v):
+                nat.call("npbx_bandwidth", mode, v, dst.data_ptr(), src.data_ptr(), nbytes, h)
+            _, ev = clock.run(fn, reps, WARM_S)
+            out[f"{name} {vname}"] = round(moved * nbytes / (ev / reps) / 1e9, 1)
+    del src, dst
+    best = {k: max(v for n, v in out.items() if n.startswith(k)) for k in ("copy", "read",
+                                                                          "write")}
+    return {"GBps": out, "best_GBps": best, "bytes": nbytes,
+            "best_frac_of_peak": {k: round(v / HBM_PEAK_GBS, 4) for k, v in best.items()}}
+
+
+def kernel_suite(st, dev, torch, clock, reps, copy_only=False, cpu=False):
+    res = {}
+    from npbench_amd import _native as nat
+
+    res["bandwidth"] = bandwidth(dev, torch, clock, reps)
+    if copy_only:
+        return res
+
+    res.update(config_suite(st, dev, torch, clock, reps, cpu))
     # ADI (polybench adi, FP64) at the reference's paper and L presets, on the
     # reference initialisation restored before each call (its values grow
     # from call to call).  Bound: the serial division chain of each line, not
@@ -334,7 +542,7 @@ def kernel_suite(st, dev, torch, clock, reps, copy_only=False):
             u.copy_(u0)
             st.adi(u, T, stream=clock.stream)
         ad()
-        _, ev = clock.run(ad, reps)
+        _, ev = clock.run(ad, reps, WARM_S)
         res[f"adi_{preset}_N{N}_T{T}_f64"] = {
             "ms_per_call": round(ev / reps * 1e3, 4),
             "glups": round(2 * T * (N - 2) ** 2 * reps / ev / 1e9, 4),
@@ -353,7 +561,7 @@ def kernel_suite(st, dev, torch, clock, reps, copy_only=False):
         b1.copy_(b0)
         st.jacobi1d(a1, b1, T1, stream=clock.stream)
     j1()
-    _, ev = clock.run(j1, reps)
+    _, ev = clock.run(j1, reps, WARM_S)
     res[f"jacobi_1d_paper_N{N1}_T{T1}_f64"] = {
         "ms_per_call": round(ev / reps * 1e3, 4),
         "glups": round(2 * (T1 - 1) * (N1 - 2) * reps / ev / 1e9, 2),
@@ -374,7 +582,7 @@ def kernel_suite(st, dev, torch, clock, reps, copy_only=False):
                 x.copy_(y)
             st.fdtd2d(*f1, TF, stream=clock.stream)
         fd()
-        _, ev = clock.run(fd, reps)
+        _, ev = clock.run(fd, reps, WARM_S)
         res[f"fdtd_2d_{preset}_{NX}x{NY}_T{TF}_f64"] = {
             "ms_per_call": round(ev / reps * 1e3, 4),
             "glups": round(TF * NX * NY * reps / ev / 1e9, 2),
@@ -397,7 +605,7 @@ def kernel_suite(st, dev, torch, clock, reps, copy_only=False):
             st.cavity_flow(NXC, NYC, NT, NIT, *c1[:2], dt, dx, dy, c1[2], 1.0, 0.1,
                            stream=clock.stream)
         cf()
-        _, ev = clock.run(cf, reps)
+        _, ev = clock.run(cf, reps, WARM_S)
         phases = NT * (NIT + 3)
         res[f"cavity_flow_{preset}_{NYC}x{NXC}_nt{NT}_nit{NIT}_f64"] = {
             "ms_per_call": round(ev / reps * 1e3, 4),
@@ -422,7 +630,7 @@ def kernel_suite(st, dev, torch, clock, reps, copy_only=False):
         nsteps.append(st.channel_flow(NITH, *h1[:2], dt, dx, dy, h1[2], 1.0, 0.1, 1.0,
                                       stream=clock.stream))
     chf()
-    _, ev = clock.run(chf, reps)
+    _, ev = clock.run(chf, reps, WARM_S)
     res[f"channel_flow_paper_{NYH}x{NXH}_nit{NITH}_f64"] = {
         "ms_per_call": round(ev / reps * 1e3, 4), "steps": nsteps[-1],
         "us_per_phase": round(ev / reps / (nsteps[-1] * (NITH + 3)) * 1e6, 3),
@@ -441,13 +649,33 @@ def kernel_suite(st, dev, torch, clock, reps, copy_only=False):
         s1.copy_(s0)
         st.seidel2d(s1, TS, stream=clock.stream)
     sd()
-    _, ev = clock.run(sd, reps)
+    _, ev = clock.run(sd, reps, WARM_S)
     res[f"seidel_2d_paper_N{NS}_T{TS}_f64"] = {
         "ms_per_call": round(ev / reps * 1e3, 4),
         "glups": round((TS - 1) * (NS - 2) ** 2 * reps / ev / 1e9, 3),
         "bound": "latency (a serial recurrence per row, rows in wavefront slots)"}
     del s0, s1
     return res
+
+
+def vadv_only(st, dev, torch, clock, reps, path):
+    """vadv 256 x 256 x 160 (FP64 and FP32 fields) alone, for profiler runs."""
+    from npbench_amd import _native as nat
+    from npbench_amd.benchmarks.weather_stencils.vadv.vadv import initialize_config as va_init
+    I, J, K = 256, 256, 160
+    dtr, *fields = va_init(I, J, K, seed=42)
+    nat.set_tuning("vadv.path", path)
+    out = {}
+    for tag, dt in (("f64", torch.float64), ("f32", torch.float32)):
+        d = [dev.to_device(a, dt) for a in fields]
+
+        def va():
+            st.vadv(*d, dtr, stream=clock.stream)
+        _, ev = clock.run(va, reps, WARM_S)
+        out[tag] = {"avg_us": round(ev / reps * 1e6, 3)}
+        del d
+    nat.set_tuning("vadv.path", 0)
+    return out
 
 
 def cpu_rates():
@@ -577,6 +805,8 @@ def only(args, st, dev, batch, tune, torch, clock, A64):
         dt = torch.float64 if args.only == "fp64" else torch.float32
         r = single_grid(st, dev, torch, clock, args, A64, dt, 0)
         return {"part": args.only, "glups": r["glups"], "avg_launch_us": r["launch_s"] * 1e6}
+    if args.only == "vadv":
+        return {"part": "vadv", **vadv_only(st, dev, torch, clock, args.steps, args.vadv_path)}
     if args.only == "serve":
         return {"part": "serve", **serving(torch, args)}
     if args.only == "copy":
@@ -591,13 +821,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=BATCH)
     ap.add_argument("--graph", type=int, default=1, help="replay a captured HIP graph (1/0)")
-    ap.add_argument("--cpu-sweeps", type=int, default=64,
-                    help="sweeps in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--cpu-sweeps", type=int, default=24,
+                    help="heat sweeps in the CPU baseline sample (0 = skip every CPU leg: "
+                         "oracle times and errors against the oracle)")
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-qps", type=float, default=400.0)
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel lines")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
-    ap.add_argument("--only", choices=["headline", "fp64", "fp32", "copy", "kernels", "serve"],
+    ap.add_argument("--vadv-path", type=int, default=0,
+                    help="with --only vadv: the vadv.path knob (0 by shape, 1 workspace, "
+                         "2 on-chip)")
+    ap.add_argument("--only", choices=["headline", "fp64", "fp32", "copy", "kernels", "serve",
+                                       "vadv"],
                     help="run one part and print its figures (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -632,7 +867,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded uniform[0,1), numpy default_rng(42))",
+        "data": "synthetic (seeded uniform[0,1), numpy default_rng(42); input sha256 "
+                f"{input_hash(A64)})",
         "config": {"workload": f"heat_3d 256^3 x {args.batch} grids, TSTEPS=101 (200 sweeps "
                                "each), FP32 storage, batched",
                    "N": HEAT_N, "tsteps": TSTEPS, "sweeps_per_grid": SWEEPS,
@@ -656,7 +892,7 @@ def main():
         line["serving"] = serving(torch, args,
                                   cpu_rates=cpu_rates() if args.cpu_sweeps > 0 else None)
     if not args.no_kernels:
-        line["kernels"] = kernel_suite(st, dev, torch, clock, 5)
+        line["kernels"] = kernel_suite(st, dev, torch, clock, 10, cpu=args.cpu_sweeps > 0)
         if args.cpu_sweeps > 0:
             adi = line["kernels"]["adi_paper_N200_T100_f64"]
             adi["cpu_oracle_ms"] = cpu_adi_ms()

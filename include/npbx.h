@@ -96,6 +96,16 @@ int npbx_cast(int src_dtype, const void *src, int dst_dtype, void *dst,
  * and the HBM traffic counters that the stencil roofline figures use. */
 int npbx_stream_copy(void *dst, const void *src, int64_t bytes, void *stream);
 
+/* Bandwidth probes over `bytes` (a multiple of 16, 16-byte aligned
+ * pointers).  No reference counterpart: they measure the device's practical
+ * HBM ceiling.  mode 0 copies src to dst, 1 reads src (dst: at least 16
+ * bytes, never written in practice), 2 writes dst (src unused).  variant
+ * picks the launch shape: one pass, a workgroup of 256 threads per 256 x U
+ * vectors, U = 4 (0), 8 (1), and with non-temporal loads and stores U = 4
+ * (2), 8 (3), 16 (4). */
+int npbx_bandwidth(int mode, int variant, void *dst, const void *src,
+                   int64_t bytes, void *stream);
+
 /* ---- 2-D 5-point Jacobi (jacobi_2d_numpy.py:4-10) -------------------------
  * A, B: N x N.  Runs TSTEPS-1 ping-pong iterations (2*(TSTEPS-1) sweeps,
  * A->B then B->A), interior only, boundaries never written.
@@ -203,7 +213,10 @@ int npbx_pack_cols(int dtype, const void *A, const void *B, int64_t n,
  * 2: 32/8, 3: 8/4), "hdiff.tblock" = iterated hdiff steps fused per HBM round
  * trip (1, the default, 2 or 4; BF16 at most 2), "adi.path" = ADI launches (0,
  * the default: by size; 1: all time steps in one single-workgroup launch, up
- * to 512 lines; 2: one launch per half step), "seidel.blocks" = wavefront
+ * to 512 lines; 2: one launch per half step), "vadv.path" = vadv kernel (0,
+ * the default: by shape; 1: c' and d' through a device workspace; 2: c' and
+ * d' in LDS where K <= 160 and columns hold whole 16-byte vectors),
+ * "seidel.blocks" = wavefront
  * slot-blocks (slots x 256-column blocks) per seidel_2d launch (default 2048,
  * about 40 ms).  Results do not depend on them. */
 int npbx_set_tuning(const char *key, int64_t value);

@@ -67,6 +67,7 @@ SIGNATURES = {
     "npbx_event_elapsed_ms": [_vp, _vp, ctypes.POINTER(ctypes.c_float)],
     "npbx_cast": [_int, _vp, _int, _vp, _i64, _vp],
     "npbx_stream_copy": [_vp, _vp, _i64, _vp],
+    "npbx_bandwidth": [_int, _int, _vp, _vp, _i64, _vp],
     "npbx_jacobi2d": [_int, _vp, _vp, _i64, _i64, _int, _vp],
     "npbx_jacobi2d_sweeps": [_int, _vp, _vp, _i64, _i64, _i64, _int, _vp],
     "npbx_heat3d": [_int, _vp, _vp, _i64, _i64, _int, _vp],

@@ -1,6 +1,8 @@
-// Streaming copy used to calibrate bandwidth figures: the measured copy rate
-// on the box is reported beside the 8 TB/s datasheet peak, and its HBM
-// counter reading scales the FETCH_SIZE/WRITE_SIZE traffic of the stencils.
+// Bandwidth calibration: copy, read-only and write-only kernels in a few
+// launch shapes, with and without non-temporal hints.  The best rate of each
+// kind on the box is reported beside the 8 TB/s datasheet peak (the practical
+// ceiling the stencils are held to), and the copy's HBM counter reading
+// scales the FETCH_SIZE/WRITE_SIZE traffic of the stencils.
 #include <algorithm>
 
 #include "common.h"
@@ -26,6 +28,91 @@ __global__ __launch_bounds__(256) void stream_copy(const uint4* __restrict__ src
   for (; i < n16; i += blockDim.x) dst[i] = src[i];
 }
 
+using u32x4 = unsigned int __attribute__((ext_vector_type(4)));
+
+template <bool NT>
+__device__ __forceinline__ uint4 ld(const uint4* p) {
+  const u32x4* q = reinterpret_cast<const u32x4*>(p);
+  u32x4 v;
+  if constexpr (NT) {
+    v = __builtin_nontemporal_load(q);
+  } else {
+    v = *q;
+  }
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+template <bool NT>
+__device__ __forceinline__ void st(uint4* p, uint4 v) {
+  u32x4* q = reinterpret_cast<u32x4*>(p);
+  const u32x4 w = {v.x, v.y, v.z, v.w};
+  if constexpr (NT) {
+    __builtin_nontemporal_store(w, q);
+  } else {
+    *q = w;
+  }
+}
+
+// One pass, no loop: workgroup b owns vectors [b*256*U, (b+1)*256*U), thread
+// t its vectors t, t+256, ... (coalesced); the last workgroup is partial.
+// MODE 0 copies, 1 reads (a store to dst[0] only if the XOR of every loaded
+// word is a value the data never gives, so nothing is written), 2 writes a
+// constant.
+template <int MODE, int U, bool NT>
+__global__ __launch_bounds__(256) void bw_pass(const uint4* __restrict__ src,
+                                               uint4* __restrict__ dst, int64_t n16) {
+  const int64_t base = (static_cast<int64_t>(blockIdx.x) * U) * 256 + threadIdx.x;
+  if constexpr (MODE == 2) {
+    const uint4 v = make_uint4(threadIdx.x, blockIdx.x, 1, 2);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = base + u * 256;
+      if (i < n16) st<NT>(dst + i, v);
+    }
+  } else {
+    uint4 r[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = base + u * 256;
+      r[u] = i < n16 ? ld<NT>(src + i) : make_uint4(0, 0, 0, 0);
+    }
+    if constexpr (MODE == 0) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t i = base + u * 256;
+        if (i < n16) st<NT>(dst + i, r[u]);
+      }
+    } else {
+      unsigned x = 0;
+#pragma unroll
+      for (int u = 0; u < U; ++u) x ^= r[u].x ^ r[u].y ^ r[u].z ^ r[u].w;
+      if (x == 0x9e3779b9u) dst[0] = r[0];
+    }
+  }
+}
+
+template <int MODE, int U, bool NT>
+int launch_pass(const void* src, void* dst, int64_t n16, hipStream_t s) {
+  const int64_t blocks = cdiv(n16, 256 * U);
+  NPBX_REQUIRE(blocks <= (int64_t(1) << 31) - 1, "bandwidth: too many workgroups");
+  hipLaunchKernelGGL((bw_pass<MODE, U, NT>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
+                     s, static_cast<const uint4*>(src), static_cast<uint4*>(dst), n16);
+  NPBX_LAUNCHED("bw_pass");
+  return NPBX_OK;
+}
+
+template <int MODE>
+int launch_variant(int variant, const void* src, void* dst, int64_t n16, hipStream_t s) {
+  switch (variant) {
+    case 0: return launch_pass<MODE, 4, false>(src, dst, n16, s);
+    case 1: return launch_pass<MODE, 8, false>(src, dst, n16, s);
+    case 2: return launch_pass<MODE, 4, true>(src, dst, n16, s);
+    case 3: return launch_pass<MODE, 8, true>(src, dst, n16, s);
+    case 4: return launch_pass<MODE, 16, true>(src, dst, n16, s);
+    default: return fail(NPBX_EINVAL, "bandwidth: variant %d out of range (0 .. 4)", variant);
+  }
+}
+
 }  // namespace
 }  // namespace npbx
 
@@ -43,4 +130,21 @@ extern "C" int npbx_stream_copy(void* dst, const void* src, int64_t bytes, void*
                      n16);
   NPBX_LAUNCHED("stream_copy");
   return NPBX_OK;
+}
+
+extern "C" int npbx_bandwidth(int mode, int variant, void* dst, const void* src, int64_t bytes,
+                              void* stream) {
+  using namespace npbx;
+  NPBX_REQUIRE(mode >= 0 && mode <= 2, "bandwidth: mode %d (0 copy, 1 read, 2 write)", mode);
+  NPBX_REQUIRE(bytes >= 0 && bytes % 16 == 0, "bandwidth: bytes must be a multiple of 16");
+  if (bytes == 0) return NPBX_OK;
+  NPBX_REQUIRE((mode == 2 || (src && aligned16(src))) && dst && aligned16(dst),
+               "bandwidth: null or misaligned pointer");
+  const int64_t n16 = bytes / 16;
+  const hipStream_t s = as_stream(stream);
+  switch (mode) {
+    case 0: return launch_variant<0>(variant, src, dst, n16, s);
+    case 1: return launch_variant<1>(variant, src, dst, n16, s);
+    default: return launch_variant<2>(variant, src, dst, n16, s);
+  }
 }

@@ -128,6 +128,10 @@ void set_hdiff_tblock(int tb);
 // single-workgroup launch, 2 = one launch per half step).
 int adi_path();
 void set_adi_path(int path);
+// vadv kernel choice ("vadv.path"; 0 = by shape, 1 = the workspace kernel,
+// 2 = the on-chip kernel where the shape allows it).
+int vadv_path();
+void set_vadv_path(int path);
 // Slot-blocks per seidel_2d launch ("seidel.blocks"; a slot-block is one
 // wavefront slot over one 256-column block).
 int seidel_blocks();

@@ -1425,6 +1425,11 @@ extern "C" int npbx_set_tuning(const char* key, int64_t value) {
     set_adi_path(static_cast<int>(value));
     return NPBX_OK;
   }
+  if (std::strcmp(key, "vadv.path") == 0) {
+    NPBX_REQUIRE(value >= 0 && value <= 2, "set_tuning: vadv.path must be 0, 1 or 2");
+    set_vadv_path(static_cast<int>(value));
+    return NPBX_OK;
+  }
   if (std::strcmp(key, "seidel.blocks") == 0) {
     NPBX_REQUIRE(value >= 1 && value <= (1 << 20),
                  "set_tuning: seidel.blocks must be in [1, 1048576]");
@@ -1433,7 +1438,7 @@ extern "C" int npbx_set_tuning(const char* key, int64_t value) {
   }
   return fail(NPBX_EINVAL,
               "set_tuning: unknown key '%s' (heat.rows, heat.seg, sweep.shape, hdiff.tblock, "
-              "adi.path, seidel.blocks)",
+              "adi.path, vadv.path, seidel.blocks)",
               key);
 }
 
@@ -1457,6 +1462,10 @@ extern "C" int npbx_get_tuning(const char* key, int64_t* value) {
   }
   if (std::strcmp(key, "hdiff.tblock") == 0) {
     *value = hdiff_tblock();
+    return NPBX_OK;
+  }
+  if (std::strcmp(key, "vadv.path") == 0) {
+    *value = vadv_path();
     return NPBX_OK;
   }
   if (std::strcmp(key, "seidel.blocks") == 0) {

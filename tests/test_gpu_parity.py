@@ -208,8 +208,12 @@ def _vadv_inputs(I, J, K, seed):
     return utens_stage, u_stage, wcon, u_pos, utens
 
 
+# K <= 160 with whole 16-byte vectors per column runs the on-chip kernel
+# (c' and d' in LDS; 40, 80 and 160 levels), other shapes the workspace
+# kernel; columns not a multiple of 64 leave a partial last wave.
 @pytest.mark.parametrize("I,J,K", [(1, 1, 2), (1, 1, 3), (5, 7, 3), (60, 60, 40),
-                                   (17, 9, 160), (3, 300, 11)])
+                                   (17, 9, 160), (3, 300, 11), (3, 37, 12), (7, 11, 80),
+                                   (5, 13, 42), (2, 64, 160), (2, 3, 200), (4, 5, 158)])
 def test_vadv_fp64_bitwise(st, I, J, K):
     print(f"vadv fp64 {I}x{J}x{K}", flush=True)
     f = _vadv_inputs(I, J, K, I + J + K)
@@ -218,6 +222,22 @@ def test_vadv_fp64_bitwise(st, I, J, K):
     d = [dev(a) for a in f]
     st.vadv(*d, 0.15)
     assert_bitwise(host(d[0]), r, "utens_stage")
+
+
+@pytest.mark.parametrize("I,J,K", [(3, 37, 12), (7, 11, 80), (5, 13, 44), (2, 64, 160),
+                                   (2, 3, 200), (4, 5, 36), (3, 5, 7)])
+def test_vadv_fp32_fields_shapes(st, I, J, K):
+    """FP32 fields through every kernel shape (on-chip 40, 80, 160 levels;
+    the workspace kernel above 160 levels or with partial vectors): within
+    the final FP32 rounding of the oracle on the same FP32 inputs."""
+    print(f"vadv fp32 fields {I}x{J}x{K}", flush=True)
+    f = _vadv_inputs(I, J, K, 7 * I + J + K)
+    q = [quantize(a, torch.float32) for a in f]
+    rq = q[0].copy()
+    ref.vadv(rq, *q[1:], 0.15)
+    d = [dev(a, torch.float32) for a in f]
+    st.vadv(*d, 0.15)
+    assert rel_inf(host(d[0]), rq) <= 1.2e-7
 
 
 @pytest.mark.parametrize("wcon_lo,f64_tol", [(0.0, 1e-5), (-1.0, None)])
