@@ -120,8 +120,12 @@ def seeded_grid(n):
 
 # ----------------------------------------------------------------- headline
 def batched_fp32(st, dev, batch, tune, torch, clock, args, A64):
-    cfg = tune.best("heat_3d", torch.float32, HEAT_N, args.batch, cache=None,
-                    sweeps=16, verbose=False)
+    if args.shape:  # a fixed launch shape (profiler runs): T, rows, seg
+        T, rows, seg = (int(x) for x in args.shape.split(","))
+        cfg = {"T": T, "rows": rows, "seg": seg, "shape": 0}
+    else:
+        cfg = tune.best("heat_3d", torch.float32, HEAT_N, args.batch, cache=None,
+                        sweeps=16, verbose=False)
     tune.apply(cfg)
     base = dev.to_device(A64, torch.float32)
     grids = [(base.clone(), base.clone()) for _ in range(args.batch)]
@@ -828,6 +832,9 @@ def main():
     ap.add_argument("--serve-qps", type=float, default=400.0)
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel lines")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
+    ap.add_argument("--shape", type=str, default=None,
+                    help="headline launch shape T,rows,seg instead of the autotuner's choice "
+                         "(profiler runs)")
     ap.add_argument("--vadv-path", type=int, default=0,
                     help="with --only vadv: the vadv.path knob (0 by shape, 1 workspace, "
                          "2 on-chip)")

@@ -224,6 +224,22 @@ def test_vadv_fp64_bitwise(st, I, J, K):
     assert_bitwise(host(d[0]), r, "utens_stage")
 
 
+@pytest.mark.parametrize("dtr,wscale", [(1e-250, 0.0), (1e-250, 1e-255), (1e250, 1.0),
+                                         (0.15, 1e-300)])
+def test_vadv_extreme_pivots(st, dtr, wscale):
+    """Pivots of magnitude about 1e-250 or 1e250 (tiny wcon leaves them at
+    dtr): bitwise the oracle at extreme exponents."""
+    I, J, K = 2, 64, 160
+    print(f"vadv extreme pivots dtr={dtr} wscale={wscale}", flush=True)
+    f = list(_vadv_inputs(I, J, K, 5))
+    f[2] = wscale * f[2]
+    r = f[0].copy()
+    ref.vadv(r, *f[1:], dtr)
+    d = [dev(a) for a in f]
+    st.vadv(*d, dtr)
+    assert_bitwise(host(d[0]), r, "utens_stage")
+
+
 @pytest.mark.parametrize("I,J,K", [(3, 37, 12), (7, 11, 80), (5, 13, 44), (2, 64, 160),
                                    (2, 3, 200), (4, 5, 36), (3, 5, 7)])
 def test_vadv_fp32_fields_shapes(st, I, J, K):
