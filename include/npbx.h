@@ -10,6 +10,15 @@
  *    kernels.  Sizes are element counts.  `stream` is a hipStream_t (NULL =
  *    the default stream); kernels are enqueued and the call returns without
  *    waiting; use npbx_sync() to wait with a deadline.
+ *  - Library scratch: iterated hdiff (nsteps > 1), vadv outside its on-chip
+ *    shapes (K > 160 or columns not whole 16-byte vectors), ADI, cavity and
+ *    channel flow, and single tiled FP32 heat grids use one grow-only
+ *    device workspace per GPU.  Calls that use it are not ordered against
+ *    each other by the library: enqueue them on one stream (or wait between
+ *    them); two such calls in flight on different streams corrupt each
+ *    other's results.  Growing the workspace synchronises the device, so a
+ *    call that grows it must not run under stream capture (run it once
+ *    outside the capture first).  npbx_adi_noalloc never touches it.
  *  - dtype selects the storage type: NPBX_F64 (the drop-in, FP64 arithmetic,
  *    bitwise equal to the reference NumPy code), NPBX_F32 and NPBX_BF16
  *    (reduced-precision storage, FP32 arithmetic).  vadv always eliminates
@@ -277,6 +286,19 @@ int npbx_channel_flow(int dtype, int64_t nit, void *u, void *v, double dt,
                       double dx, double dy, void *p, double rho, double nu,
                       double F, int64_t nx, int64_t ny, int64_t max_steps,
                       int64_t *stepcount, void *stream);
+
+/* One bounded step of npbx_channel_flow: a single launch of at most
+ * max_steps time steps continuing from u, v, p, then a stream
+ * synchronisation; *steps = the steps it ran, *converged = 1 when the
+ * reference's criterion stopped it.  Repeating it until *converged is
+ * npbx_channel_flow, with control back in the caller between launches (so
+ * a caller's timeout can stop an input that never converges, where the
+ * reference's loop runs until interrupted). */
+int npbx_channel_flow_launch(int dtype, int64_t nit, void *u, void *v,
+                             double dt, double dx, double dy, void *p,
+                             double rho, double nu, double F, int64_t nx,
+                             int64_t ny, int64_t max_steps, int64_t *steps,
+                             int32_t *converged, void *stream);
 
 /* ---- 2-D Gauss-Seidel (seidel_2d_numpy.py:4-13) ----------------------------
  * A: (N, N) FP64 (dtype NPBX_F64), updated in place by TSTEPS - 1 sweeps;

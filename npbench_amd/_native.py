@@ -100,6 +100,10 @@ SIGNATURES = {
     "npbx_channel_flow": [_int, _i64, _vp, _vp, ctypes.c_double, ctypes.c_double,
                           ctypes.c_double, _vp, ctypes.c_double, ctypes.c_double,
                           ctypes.c_double, _i64, _i64, _i64, ctypes.POINTER(_i64), _vp],
+    "npbx_channel_flow_launch": [_int, _i64, _vp, _vp, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_double, _vp, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_double, _i64, _i64, _i64, ctypes.POINTER(_i64),
+                                 ctypes.POINTER(ctypes.c_int32), _vp],
     "npbx_seidel2d": [_int, _vp, _i64, _i64, _vp],
     "npbx_adi_division_check": [_i64, ctypes.c_uint64, ctypes.POINTER(_i64),
                                 ctypes.POINTER(_i64)],

@@ -393,11 +393,14 @@ __global__ __launch_bounds__(kThreads) void channel_flow_lds(ChannelArgs g) {
 }  // namespace
 }  // namespace npbx
 
-extern "C" int npbx_channel_flow(int dtype, int64_t nit, void* u, void* v, double dt, double dx,
-                                 double dy, void* p, double rho, double nu, double F, int64_t nx,
-                                 int64_t ny, int64_t max_steps, int64_t* stepcount,
-                                 void* stream) {
-  using namespace npbx;
+namespace npbx {
+namespace {
+
+// Checks the arguments and fills the launch arguments (scratch from the
+// library workspace).
+int channel_setup(int dtype, int64_t nit, void* u, void* v, double dt, double dx, double dy,
+                  void* p, double rho, double nu, double F, int64_t nx, int64_t ny,
+                  int64_t max_steps, ChannelArgs& g) {
   NPBX_REQUIRE(dtype == NPBX_F64, "channel_flow: dtype %s not supported (f64)",
                dtype_name(dtype));
   NPBX_REQUIRE(nx >= 3 && ny >= 3 && nx * ny <= (int64_t(1) << 26),
@@ -406,15 +409,12 @@ extern "C" int npbx_channel_flow(int dtype, int64_t nit, void* u, void* v, doubl
   NPBX_REQUIRE(nit >= 0 && nit <= (int64_t(1) << 30), "channel_flow: nit out of range");
   NPBX_REQUIRE(max_steps >= 1 && max_steps <= (int64_t(1) << 30),
                "channel_flow: max_steps out of range (1 .. 2^30)");
-  NPBX_REQUIRE(stepcount != nullptr, "channel_flow: null stepcount");
   NPBX_REQUIRE(u && v && p && u != v && v != p && u != p,
                "channel_flow: u, v, p must be distinct non-null arrays");
   const int64_t np = nx * ny;
-  const hipStream_t s = as_stream(stream);
   void* ws = nullptr;
   const int64_t stat_off = 4 * np, red_off = stat_off + 64;
   NPBX_TRY(workspace(static_cast<size_t>(red_off + kWaves) * sizeof(double), &ws));
-  ChannelArgs g;
   g.u = static_cast<double*>(u);
   g.v = static_cast<double*>(v);
   g.p = static_cast<double*>(p);
@@ -448,23 +448,62 @@ extern "C" int npbx_channel_flow(int dtype, int64_t nit, void* u, void* v, doubl
   g.cdx2 = dt / g.dx2;
   g.cdy2 = dt / g.dy2;
   g.fdt = F * dt;
+  g.max_steps = static_cast<int>(max_steps);
+  return NPBX_OK;
+}
+
+// One launch of at most g.max_steps time steps, continuing from u, v, p; waits
+// for it and returns the steps it ran and whether the criterion stopped it.
+int channel_launch(const ChannelArgs& g, hipStream_t s, int64_t* steps, bool* done) {
+  if (static_cast<int64_t>(g.nx) * g.ny <= kLdsPoints) {
+    hipLaunchKernelGGL(channel_flow_lds, dim3(1), dim3(kThreads), 0, s, g);
+    NPBX_LAUNCHED("channel_flow_lds");
+  } else {
+    hipLaunchKernelGGL(channel_flow_global, dim3(1), dim3(kThreads), 0, s, g);
+    NPBX_LAUNCHED("channel_flow_global");
+  }
+  double stat[2];
+  NPBX_HIP(hipMemcpyAsync(stat, g.stat, sizeof(stat), hipMemcpyDeviceToHost, s));
+  NPBX_HIP(hipStreamSynchronize(s));
+  *steps = static_cast<int64_t>(stat[0]);
+  *done = stat[1] != 0.0;
+  return NPBX_OK;
+}
+
+}  // namespace
+}  // namespace npbx
+
+extern "C" int npbx_channel_flow(int dtype, int64_t nit, void* u, void* v, double dt, double dx,
+                                 double dy, void* p, double rho, double nu, double F, int64_t nx,
+                                 int64_t ny, int64_t max_steps, int64_t* stepcount,
+                                 void* stream) {
+  using namespace npbx;
+  NPBX_REQUIRE(stepcount != nullptr, "channel_flow: null stepcount");
+  ChannelArgs g;
+  NPBX_TRY(channel_setup(dtype, nit, u, v, dt, dx, dy, p, rho, nu, F, nx, ny, max_steps, g));
   // launches of at most max_steps steps until the criterion stops one
   int64_t total = 0;
   for (;;) {
-    g.max_steps = static_cast<int>(max_steps);
-    if (np <= kLdsPoints) {
-      hipLaunchKernelGGL(channel_flow_lds, dim3(1), dim3(kThreads), 0, s, g);
-      NPBX_LAUNCHED("channel_flow_lds");
-    } else {
-      hipLaunchKernelGGL(channel_flow_global, dim3(1), dim3(kThreads), 0, s, g);
-      NPBX_LAUNCHED("channel_flow_global");
-    }
-    double stat[2];
-    NPBX_HIP(hipMemcpyAsync(stat, g.stat, sizeof(stat), hipMemcpyDeviceToHost, s));
-    NPBX_HIP(hipStreamSynchronize(s));
-    total += static_cast<int64_t>(stat[0]);
-    if (stat[1] != 0.0) break;
+    int64_t steps = 0;
+    bool done = false;
+    NPBX_TRY(channel_launch(g, as_stream(stream), &steps, &done));
+    total += steps;
+    if (done) break;
   }
   *stepcount = total;
+  return NPBX_OK;
+}
+
+extern "C" int npbx_channel_flow_launch(int dtype, int64_t nit, void* u, void* v, double dt,
+                                        double dx, double dy, void* p, double rho, double nu,
+                                        double F, int64_t nx, int64_t ny, int64_t max_steps,
+                                        int64_t* steps, int32_t* converged, void* stream) {
+  using namespace npbx;
+  NPBX_REQUIRE(steps != nullptr && converged != nullptr, "channel_flow_launch: null output");
+  ChannelArgs g;
+  NPBX_TRY(channel_setup(dtype, nit, u, v, dt, dx, dy, p, rho, nu, F, nx, ny, max_steps, g));
+  bool done = false;
+  NPBX_TRY(channel_launch(g, as_stream(stream), steps, &done));
+  *converged = done ? 1 : 0;
   return NPBX_OK;
 }

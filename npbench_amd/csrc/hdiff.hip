@@ -101,17 +101,32 @@ __global__ __launch_bounds__(kLanes* kCols) void hdiff_kernel(const S* __restric
 }
 
 // Copies the 2-cell halo ring (everything outside the interior) of src into
-// dst; both are (I+4, J+4, K).
+// dst; both are (I+4, J+4, K).  Only the ring's elements are visited: the
+// columns (p, q) of rows p < 2 and p >= I+2 (whole rows) and of columns
+// q < 2 and q >= J+2 of the rows between, each column's K elements
+// contiguous.
 template <class S>
 __global__ __launch_bounds__(256) void hdiff_copy_halo(const S* __restrict__ src,
                                                        S* __restrict__ dst, int I, int J, int K) {
-  const int64_t total = static_cast<int64_t>(I + 4) * (J + 4) * K;
+  const int64_t row_cols = 4 * static_cast<int64_t>(J + 4);  // rows 0, 1, I+2, I+3
+  const int64_t side_cols = 4 * static_cast<int64_t>(I);     // 4 columns of rows 2 .. I+1
+  const int64_t total = (row_cols + side_cols) * K;
   for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
        e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const int64_t col = e / K;
-    const int p = static_cast<int>(col / (J + 4));
-    const int q = static_cast<int>(col % (J + 4));
-    if (p < 2 || p >= I + 2 || q < 2 || q >= J + 2) dst[e] = src[e];
+    const int64_t c = e / K, k = e % K;
+    int64_t p, q;
+    if (c < row_cols) {
+      const int64_t r = c / (J + 4);
+      p = r < 2 ? r : I + r;  // 0, 1, I+2, I+3
+      q = c % (J + 4);
+    } else {
+      const int64_t s = c - row_cols;
+      p = 2 + s / 4;
+      const int64_t w = s % 4;
+      q = w < 2 ? w : J + w;  // 0, 1, J+2, J+3
+    }
+    const int64_t at = (p * (J + 4) + q) * K + k;
+    dst[at] = src[at];
   }
 }
 
@@ -399,7 +414,8 @@ int run_hdiff(const S* in, S* out, const S* coeff, int I, int J, int K, int64_t 
   void* ws = nullptr;
   NPBX_TRY(workspace(2 * field * sizeof(S), &ws));
   S* w[2] = {static_cast<S*>(ws), static_cast<S*>(ws) + field};
-  const unsigned copy_blocks = static_cast<unsigned>(std::min<int64_t>(cdiv(field, 256), 4096));
+  const int64_t ring = (4 * static_cast<int64_t>(J + 4) + 4 * static_cast<int64_t>(I)) * K;
+  const unsigned copy_blocks = static_cast<unsigned>(std::min<int64_t>(cdiv(ring, 256), 4096));
   for (int b = 0; b < 2; ++b) {
     hipLaunchKernelGGL(hdiff_copy_halo<S>, dim3(copy_blocks), dim3(256), 0, stream, in, w[b], I, J,
                        K);

@@ -127,19 +127,27 @@ def cavity_flow(nx, ny, nt, nit, u, v, dt, dx, dy, p, rho, nu, stream=None):
 
 def channel_flow(nit, u, v, dt, dx, dy, p, rho, nu, F, stream=None, max_steps=None):
     """channel_flow_numpy.channel_flow on float64 (ny, nx) device tensors u,
-    v, p (in place); returns the step count.  Synchronises the stream (the
-    number of steps depends on the data).  max_steps bounds the steps of one
-    kernel launch; by default a launch holds about 50 ms of work."""
+    v, p (in place); returns the step count.  Runs launches of at most
+    max_steps steps (by default about 50 ms of work each), synchronising
+    after each, until the reference's criterion stops one; control returns
+    to Python between launches, so a harness timeout (an interrupt of the
+    main thread) stops an input that never converges, as it stops the
+    reference's unbounded loop."""
     ny, nx = (int(d) for d in u.shape)
     for name, t in (("u", u), ("v", v), ("p", p)):
         dev.check_device_array(name, t, shape=(ny, nx), dtype=torch.float64)
     if max_steps is None:
         max_steps = max(1, min(4096, 150_000_000 // (nx * ny * (int(nit) + 3))))
     steps = ctypes.c_int64(0)
-    nat.call("npbx_channel_flow", dev.dtype_code(u), int(nit), dev.ptr(u), dev.ptr(v), float(dt),
-             float(dx), float(dy), dev.ptr(p), float(rho), float(nu), float(F), nx, ny,
-             int(max_steps), ctypes.byref(steps), dev.stream_handle(stream))
-    return steps.value
+    done = ctypes.c_int32(0)
+    total = 0
+    while not done.value:
+        nat.call("npbx_channel_flow_launch", dev.dtype_code(u), int(nit), dev.ptr(u), dev.ptr(v),
+                 float(dt), float(dx), float(dy), dev.ptr(p), float(rho), float(nu), float(F),
+                 nx, ny, int(max_steps), ctypes.byref(steps), ctypes.byref(done),
+                 dev.stream_handle(stream))
+        total += steps.value
+    return total
 
 
 def seidel2d(A, TSTEPS, stream=None):

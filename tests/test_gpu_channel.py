@@ -84,3 +84,46 @@ def test_channel_dropin_numpy_roundtrip(st):
     assert channel_flow_mi355x.channel_flow(3, u, v, dt, dx, dy, p, 1.0, 0.1, 1.0) == steps
     for g, w, name in zip((u, v, p), want, ("u", "v", "p")):
         assert_bitwise(g, w, name)
+
+
+def test_channel_c_loop_entry_point(st):
+    """npbx_channel_flow (the whole loop inside the library, for C callers)
+    gives the oracle's step count and fields, like the launch-by-launch
+    path."""
+    from npbench_amd import _native, device
+    import ctypes
+    ny, nx, nit = 40, 37, 3
+    dx, dy, dt = 2 / (nx - 1), 2 / (ny - 1), .1 / ((nx - 1) * (ny - 1))
+    rng = np.random.default_rng(12)
+    u, v, p = np.zeros((ny, nx)), 0.01 * rng.random((ny, nx)), 1 + 0.1 * rng.random((ny, nx))
+    want = [u.copy(), v.copy(), p.copy()]
+    steps = ref.channel_flow(nit, want[0], want[1], dt, dx, dy, want[2], 1.0, 0.1, 1.0)
+    d = [device.to_device(a, torch.float64) for a in (u, v, p)]
+    got = ctypes.c_int64(0)
+    print("channel_flow C loop 40x37", flush=True)
+    _native.call("npbx_channel_flow", _native.F64, nit, device.ptr(d[0]), device.ptr(d[1]), dt,
+                 dx, dy, device.ptr(d[2]), 1.0, 0.1, 1.0, nx, ny, 5, ctypes.byref(got),
+                 device.stream_handle(None))
+    assert got.value == steps
+    for name, g, w in zip(("u", "v", "p"), d, want):
+        assert_bitwise(g.cpu().numpy(), w, name)
+
+
+def test_channel_flow_yields_to_a_harness_timeout(st):
+    """The harness's timeout (an interrupt of the main thread) stops a long
+    channel flow between two bounded launches: one step of 500 pressure
+    iterations per launch on the paper grid (about 2 ms a launch, on the
+    order of a thousand launches to convergence) under a 0.3 s limit."""
+    from npbench_amd import device
+    from npbench_amd.infrastructure import timeout
+    from npbench_amd.benchmarks.channel_flow import channel_flow
+    u, v, p, dx, dy, dt = channel_flow.initialize(101, 101)
+    d = [device.to_device(a, torch.float64) for a in (u, v, p)]
+
+    @timeout.exit_after(0.3)
+    def run():
+        return st.channel_flow(500, d[0], d[1], dt, dx, dy, d[2], 1.0, 0.1, 1.0, max_steps=1)
+    print("channel_flow under a 0.3 s timeout", flush=True)
+    with pytest.raises(KeyboardInterrupt):
+        run()
+    torch.cuda.synchronize()
