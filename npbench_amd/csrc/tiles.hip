@@ -888,18 +888,31 @@ __global__ __launch_bounds__(256) void pack_cols_kernel(const float* __restrict_
   int* flag = reinterpret_cast<int*>(cols + 4 * nn);
   bool same = true;
   auto eq = [](float x, float y) { return __float_as_uint(x) == __float_as_uint(y); };
-  for (int64_t t = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; t < nn;
-       t += static_cast<int64_t>(gridDim.x) * 256) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * 256;
+  const int64_t t0 = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  // the two boundary columns of every row, packed (and compared)
+  for (int64_t t = t0; t < nn; t += stride) {
     const int64_t row = t * n;  // start of row j of plane q, t = q*n + j
     const float a0 = A[row], a1 = A[row + n - 1], b0 = B[row], b1 = B[row + n - 1];
     cols[t * 2] = a0;
     cols[t * 2 + 1] = a1;
     cols[(nn + t) * 2] = b0;
     cols[(nn + t) * 2 + 1] = b1;
-    same = same && eq(a0, b0) && eq(a1, b1);
-    const int64_t q = t / n, j = t % n;
-    if (q == 0 || q == n - 1 || j == 0 || j == n - 1)  // a whole boundary row
-      for (int64_t k = 1; k < n - 1; ++k) same = same && eq(A[row + k], B[row + k]);
+    same = same & eq(a0, b0) & eq(a1, b1);
+  }
+  // the rest of the boundary faces, element by element along k (coalesced):
+  // planes 0 and n-1 whole, rows 0 and n-1 of the planes between
+  const int64_t planes = 2 * nn, rows = 2 * static_cast<int64_t>(n - 2) * n;
+  for (int64_t e = t0; e < planes + rows; e += stride) {
+    int64_t at;
+    if (e < planes) {
+      at = (e < nn ? 0 : (static_cast<int64_t>(n) - 1) * nn) + e % nn;
+    } else {
+      const int64_t r = e - planes, q = 1 + r / (2 * static_cast<int64_t>(n));
+      const int64_t side = (r / n) % 2, k = r % n;
+      at = q * nn + (side ? static_cast<int64_t>(n - 1) * n : 0) + k;
+    }
+    same = same & eq(A[at], B[at]);
   }
   if (!same) *flag = 0;  // every writer stores the same value
 }
@@ -911,7 +924,7 @@ int pack_cols(const void* A, const void* B, int n, void* cols, hipStream_t strea
   int* flag = reinterpret_cast<int*>(static_cast<float*>(cols) + 4 * nn);
   hipLaunchKernelGGL(pack_flag_kernel, dim3(1), dim3(1), 0, stream, flag);
   NPBX_LAUNCHED("pack_flag");
-  const unsigned blocks = static_cast<unsigned>(std::min<int64_t>(cdiv(nn, 256), 4096));
+  const unsigned blocks = static_cast<unsigned>(std::min<int64_t>(cdiv(4 * nn, 256), 4096));
   hipLaunchKernelGGL(pack_cols_kernel, dim3(blocks), dim3(256), 0, stream,
                      static_cast<const float*>(A), static_cast<const float*>(B),
                      static_cast<float*>(cols), n);
