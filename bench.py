@@ -369,9 +369,14 @@ def config_suite(st, dev, torch, clock, reps, cpu):
                 line["max_rel_err_vs_fp64_oracle"] = max(x["inf_norm"] for x in e)
                 line["error_vs_fp64_oracle"] = {k: max(x[k] for x in e) for k in e[0]}
         wall, ev = clock.run(jac, reps, WARM_S)
+        # the kernel stages 8 sweeps per launch in LDS, so a sweep's own
+        # read + write (2 x element x points) over 8 TB/s is a
+        # sweep-equivalent rate, not HBM traffic
         line.update({"glups": round(lups * reps / ev / 1e9, 2),
                      "avg_sweep_us": round(ev / reps / SWEEPS * 1e6, 3),
-                     "hbm_frac": round(2 * SIZE[tag] * lups * reps / ev / 1e9 / HBM_PEAK_GBS, 4),
+                     "sweeps_per_launch": int(nat.get_tuning("jacobi.deep")),
+                     "sweep_equiv_bw_frac": round(2 * SIZE[tag] * lups * reps / ev / 1e9
+                                                  / HBM_PEAK_GBS, 4),
                      "input_sha256_16": input_hash(A, B)})
         if cpu:
             line["speedup_vs_cpu_oracle"] = round(

@@ -227,7 +227,10 @@ int npbx_pack_cols(int dtype, const void *A, const void *B, int64_t n,
  * d' in LDS where K <= 160 and columns hold whole 16-byte vectors),
  * "seidel.blocks" = wavefront
  * slot-blocks (slots x 256-column blocks) per seidel_2d launch (default 2048,
- * about 40 ms).  Results do not depend on them. */
+ * about 40 ms), "jacobi.deep" = 2-D Jacobi sweeps per launch at tblock <= 1
+ * (default 8: LDS-blocked, 64 x 48 points per workgroup; 0 or 1: one launch
+ * per sweep), "jacobi.deep_waves" = its waves per workgroup (0, the default:
+ * by shape; 4, 8 or 16).  Results do not depend on them. */
 int npbx_set_tuning(const char *key, int64_t value);
 int npbx_get_tuning(const char *key, int64_t *value);
 

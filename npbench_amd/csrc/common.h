@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "npbx.h"
 
@@ -116,6 +117,21 @@ int dispatch_dtype(int dtype, F&& f) {
 // through the tile kernel: depth T (>= 2) per HBM round trip.
 int tiled_sweeps(int kind, int dtype, void* A, void* B, int n, int64_t first, int64_t nsweeps,
                  int T, int seg, hipStream_t stream);
+
+// Depths of the rounds that advance nsweeps sweeps at depth <= T, the last
+// two levels ending in A and B as in the reference (tiles.hip).
+std::vector<int> sweep_rounds(int64_t nsweeps, int T);
+
+// 2-D Jacobi temporal-blocked in LDS (jacobi2d_deep.hip): sweeps per launch
+// ("jacobi.deep"; 0 or 1 = one launch per sweep), waves per workgroup
+// ("jacobi.deep_waves") and the sweeps themselves.
+int jacobi_deep_depth();
+void set_jacobi_deep_depth(int d);
+int jacobi_deep_waves();
+void set_jacobi_deep_waves(int w);
+int jacobi_deep_max();
+int jacobi_deep_sweeps(int dtype, void* A, void* B, int n, int64_t first, int64_t nsweeps, int T,
+                       hipStream_t stream);
 
 // Plain-sweep shape chosen by the tuning knob "sweep.shape" (0 = 16 planes
 // per workgroup with loads of 4 planes in flight, 1 = 16/8, 2 = 32/8,

@@ -154,6 +154,10 @@ int jacobi_entry(int dtype, void* A, void* B, int64_t N, int64_t first, int64_t 
   hipStream_t s = as_stream(stream);
   if (tblock >= 2)
     return tiled_sweeps(0, dtype, A, B, static_cast<int>(N), first, nsweeps, tblock, 0, s);
+  NPBX_REQUIRE(dtype_size(dtype) != 0, "jacobi2d: unknown dtype %d", dtype);
+  const int deep = jacobi_deep_depth();
+  if (deep >= 2 && nsweeps >= 2)
+    return jacobi_deep_sweeps(dtype, A, B, static_cast<int>(N), first, nsweeps, deep, s);
   return dispatch_dtype(dtype, [&](auto tag) {
     using S = decltype(tag);
     return run_sweeps<S>(static_cast<S*>(A), static_cast<S*>(B), static_cast<int>(N), first,

@@ -886,8 +886,8 @@ __global__ __launch_bounds__(256) void pack_cols_kernel(const float* __restrict_
                                                         float* __restrict__ cols, int n) {
   const int64_t nn = static_cast<int64_t>(n) * n;
   int* flag = reinterpret_cast<int*>(cols + 4 * nn);
-  bool same = true;
-  auto eq = [](float x, float y) { return __float_as_uint(x) == __float_as_uint(y); };
+  unsigned same = 1;  // bitwise and of every comparison: no branch between the loads
+  auto eq = [](float x, float y) { return __float_as_uint(x) == __float_as_uint(y) ? 1u : 0u; };
   const int64_t stride = static_cast<int64_t>(gridDim.x) * 256;
   const int64_t t0 = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
   // the two boundary columns of every row, packed (and compared)
@@ -1059,6 +1059,8 @@ int resident_blocks(int dtype, int T, int RH) {
 }  // namespace
 
 int sweep_shape() { return g_sweep_shape.load(); }
+
+std::vector<int> sweep_rounds(int64_t nsweeps, int T) { return round_depths(nsweeps, T); }
 
 // Single grid, temporal-blocked: rounds of depth <= T through the tile
 // kernel (one workgroup per tile), then the final depth-1 round.
@@ -1438,6 +1440,18 @@ extern "C" int npbx_set_tuning(const char* key, int64_t value) {
     set_adi_path(static_cast<int>(value));
     return NPBX_OK;
   }
+  if (std::strcmp(key, "jacobi.deep") == 0) {
+    NPBX_REQUIRE(value >= 0 && value <= jacobi_deep_max(),
+                 "set_tuning: jacobi.deep must be in [0, %d]", jacobi_deep_max());
+    set_jacobi_deep_depth(static_cast<int>(value));
+    return NPBX_OK;
+  }
+  if (std::strcmp(key, "jacobi.deep_waves") == 0) {
+    NPBX_REQUIRE(value == 0 || value == 4 || value == 8 || value == 16,
+                 "set_tuning: jacobi.deep_waves must be 0 (by shape), 4, 8 or 16");
+    set_jacobi_deep_waves(static_cast<int>(value));
+    return NPBX_OK;
+  }
   if (std::strcmp(key, "vadv.path") == 0) {
     NPBX_REQUIRE(value >= 0 && value <= 2, "set_tuning: vadv.path must be 0, 1 or 2");
     set_vadv_path(static_cast<int>(value));
@@ -1451,7 +1465,7 @@ extern "C" int npbx_set_tuning(const char* key, int64_t value) {
   }
   return fail(NPBX_EINVAL,
               "set_tuning: unknown key '%s' (heat.rows, heat.seg, sweep.shape, hdiff.tblock, "
-              "adi.path, vadv.path, seidel.blocks)",
+              "adi.path, vadv.path, seidel.blocks, jacobi.deep, jacobi.deep_waves)",
               key);
 }
 
@@ -1475,6 +1489,14 @@ extern "C" int npbx_get_tuning(const char* key, int64_t* value) {
   }
   if (std::strcmp(key, "hdiff.tblock") == 0) {
     *value = hdiff_tblock();
+    return NPBX_OK;
+  }
+  if (std::strcmp(key, "jacobi.deep") == 0) {
+    *value = jacobi_deep_depth();
+    return NPBX_OK;
+  }
+  if (std::strcmp(key, "jacobi.deep_waves") == 0) {
+    *value = jacobi_deep_waves();
     return NPBX_OK;
   }
   if (std::strcmp(key, "vadv.path") == 0) {
