@@ -48,14 +48,19 @@ struct bf16 {
   uint16_t bits;
 };
 
-// Round-to-nearest-even float -> bf16 from the IEEE-754 definition: add
-// half an ulp of the target (minus one when the kept lsb is 0, so ties go to
-// even) and truncate.  NaNs stay NaN (quiet bit forced).
+// Round-to-nearest-even float -> bf16 by the CDNA4 conversion instruction
+// (v_cvt_pk_bf16_f32): bitwise the IEEE-754 definition (add half an ulp of
+// the target, minus one when the kept lsb is 0, and truncate) for every
+// non-NaN float, denormals included (exhaustive GPU test); NaN -> a quiet NaN.
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ uint16_t f32_to_bf16_bits(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<uint16_t>((u >> 16) | 0x0040u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return static_cast<uint16_t>(u >> 16);
+  return __builtin_bit_cast(uint16_t, static_cast<__bf16>(f));
+}
+// Two conversions in one instruction: a in the low half, b in the high.
+__device__ __forceinline__ uint32_t f32x2_to_bf16x2_bits(float a, float b) {
+  const f32x2_t v = {a, b};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2_t));
 }
 
 template <class S>
@@ -140,6 +145,11 @@ int sweep_shape();
 // Fused hdiff steps per round ("hdiff.tblock"; 1 = one launch per step).
 int hdiff_tblock();
 void set_hdiff_tblock(int tb);
+// hdiff marching shape ("hdiff.shape"; 0 = by storage type, 1 .. 13 =
+// {columns per lane, rows per thread, bytes of k per lane}, hdiff.hip).
+int hdiff_shape();
+void set_hdiff_shape(int shape);
+int hdiff_shape_count();
 // ADI launch path ("adi.path"; 0 = by size, 1 = every time step in one
 // single-workgroup launch, 2 = one launch per half step).
 int adi_path();

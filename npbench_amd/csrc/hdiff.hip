@@ -6,18 +6,16 @@
 // The stencil acts in (i, j) only; every k is independent.  Output point
 // (x, y, k) sits over in-field point (x+2, y+2, k).
 //
-// Marching kernel (hdiff_march): a lane owns one output column y and one
-// vector of V consecutive k (16 bytes; 8 for BF16, see march_vec) (lanes
-// enumerate (y, k-vector) pairs,
-// so any K maps densely onto waves and loads stay contiguous), and walks
-// down a strip of march_strip<S>() output rows x.  It keeps in registers the in-field
-// rows p-2..p+2 of its column q, rows p-1..p+1 of columns q+-1 and row p of
-// q+-2 (p = x+2, q = y+2), and carries lap(p-1), lap(p) and the i-flux of
-// the face below from one row to the next, so each output point loads five
-// new vectors and computes three Laplacians and three fluxes.  The next
-// row's loads are issued before the current row is computed.  The 2-cell
-// halo of the input makes every access in bounds.  Same operations in the
-// same order as the reference, so FP64 is bitwise NumPy.
+// Marching kernel (hdiff_march): a lane owns W adjacent output columns and
+// one vector of V consecutive k (4, 8 or 16 bytes by shape)
+// (lanes enumerate (column group, k-vector) pairs, so any K maps densely
+// onto waves and loads stay contiguous), and walks down a strip of output
+// rows x, keeping the in-field rows it still needs in registers and
+// carrying the Laplacians and the i-flux of the face below from one row to
+// the next; the next row's loads are issued before the current row is
+// computed.  The 2-cell halo of the input makes every access in bounds.
+// Same operations in the same order as the reference, so FP64 is bitwise
+// NumPy.
 //
 // Fallback (hdiff_kernel, K not a multiple of V or unaligned arrays): one
 // output point per thread, the Laplacians rebuilt from 13 cached loads.
@@ -130,130 +128,221 @@ __global__ __launch_bounds__(256) void hdiff_copy_halo(const S* __restrict__ src
   }
 }
 
-// Output rows per thread: 16, and 8 for BF16, whose loads carry half the
-// bytes and which needs the doubled number of waves to hide their latency
-// (FP32 measured slower with 8).
-template <class S>
-constexpr int march_strip() { return sizeof(S) == 2 ? 8 : 16; }
-
-// k-elements per lane of the marching kernel: one 16-byte vector, except
-// BF16, which takes 4 (8 bytes) so that it runs as many waves with the same
-// register footprint as FP32 (8 would halve the waves and double the window).
-template <class S>
-constexpr int march_vec() { return sizeof(S) == 2 ? 4 : Vec16<S>::N; }
-
-template <class S>
+// hdiff_march<S, W, STRIP, VB>: a lane owns W adjacent output columns y0
+// .. y0+W-1 at one k-vector of VB bytes (V = VB / sizeof(S) elements) and
+// marches down STRIP output rows.  Per row it
+// loads the W own columns' row p+2, the side columns' (y0-1, y0+W) row p+1
+// and the far columns' (y0-2, y0+W+1) row p -- W + 4 vectors for W points --
+// and computes W Laplacians one row down (carried to the next row), the two
+// side columns' Laplacians, W i-fluxes (the face below is carried) and W + 1
+// j-fluxes: the j-flux between two own columns is one value, the Laplacian
+// of an own column is the one its neighbour needs.  Every Laplacian and
+// flux has the reference's operands in the reference's order, whoever
+// computes it, so any W gives bitwise the same output.  The last column
+// group is shifted left to end at column J-1 (J >= W) and stores only its
+// own columns, so every column is stored once and every load is in the
+// field.  Prefetched vectors are held as raw storage words until the row
+// that uses them.
+template <class S, int W, int STRIP, int VB>
 __global__ __launch_bounds__(256) void hdiff_march(const S* __restrict__ in, S* __restrict__ out,
                                                    const S* __restrict__ coeff, int I, int J,
                                                    int K, int xblocks, S* __restrict__ halo_dst) {
   using T = Storage<S>;
   using C = typename T::C;
-  constexpr int V = march_vec<S>();
+  constexpr int V = VB / static_cast<int>(sizeof(S));
+  static_assert(V >= 1, "a vector holds at least one element");
+  using Raw = RawVec<S, V>;
   const int b = xcd_tile(blockIdx.x, gridDim.x);
   const int xb = b % xblocks, strip = b / xblocks;
   const int kvs = K / V;
+  const int groups = (J + W - 1) / W;
   const int t = xb * 256 + threadIdx.x;
-  if (t >= J * kvs) return;
-  const int y = t / kvs, k0 = (t % kvs) * V;
-  constexpr int kStrip = march_strip<S>();
-  const int x0 = strip * kStrip, x1 = min(x0 + kStrip, I);
+  if (t >= groups * kvs) return;
+  const int yn = (t / kvs) * W;     // the group's first column
+  const int y0 = min(yn, J - W);    // first column computed
+  const int k0 = (t % kvs) * V;
+  const int x0 = strip * STRIP, x1 = min(x0 + STRIP, I);
   const int64_t si = static_cast<int64_t>(J + 4) * K;  // in-field row stride
-  const int64_t sj = K;
-  const S* f = in + (y + 2) * sj + k0;                // column q = y+2, row 0
-  auto ld = [&](int p, int dq, C (&v)[V]) { loadv<S, V>(f + p * si + dq * sj, v); };
-
-  // window at row p = x0+2: c0[0..4] = rows p-2..p+2 (col q), cm/cp[0..2] =
-  // rows p-1..p+1 (cols q-1 / q+1), cmm / cpp = row p (cols q-2 / q+2)
-  C c0[5][V], cm[3][V], cp[3][V], cmm[V], cpp[V];
-  const int p0 = x0 + 2;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) ld(p0 - 2 + r, 0, c0[r]);
-#pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    ld(p0 - 1 + r, -1, cm[r]);
-    ld(p0 - 1 + r, 1, cp[r]);
-  }
-  // the loads that complete the window of row p0 (and start each next row)
-  C n0[V], nm[V], np[V], nmm[V], npp[V], cf[V];
-  ld(p0 + 2, 0, n0);
-  ld(p0 + 1, -1, nm);
-  ld(p0 + 1, 1, np);
-  ld(p0, -2, nmm);
-  ld(p0, 2, npp);
-  const S* cfp = coeff + (static_cast<int64_t>(x0) * J + y) * K + k0;
-  loadv<S, V>(cfp, cf);
-
+  // in-field column y0 + c, c = 0 .. W+3: far left, side left, the W own
+  // columns, side right, far right
+  const S* __restrict__ base = in + static_cast<int64_t>(y0) * K + k0;
+  auto ld = [&](int p, int c, Raw& r) { loadraw<S, V>(base + p * si + c * K, r); };
   auto lap = [](C c, C ip, C im, C jp, C jm) {
     C nb = ip + im;
     nb = nb + jp;
     nb = nb + jm;
     return static_cast<C>(4.0) * c - nb;
   };
-  C L0[V], flx_lo[V];  // carried: lap(p) and the flux across (p-1 | p)
-  for (int x = x0; x < x1; ++x) {
-    const int p = x + 2;
+
+  // Row slots: the own columns' rows p, p+1, p+2 and the side columns' (0
+  // left, 1 right) rows p-1, p, p+1 sit in three slots each, which the
+  // unrolled march rotates instead of moving registers; far[f] is far
+  // column f at row p.
+  using Own = C[W][V];
+  using Side = C[2][V];
+  Own oa, ob, oc;
+  Side sa, sb, sc;
+  C far[2][V];
+  C L0[W][V], fxl[W][V];  // carried: lap(p) and the i-flux across (p-1 | p)
+  const int p0 = x0 + 2;
+  {
+    Raw r[W][4], rs[2][2];
 #pragma unroll
-    for (int v = 0; v < V; ++v) {
-      c0[4][v] = n0[v];
-      cm[2][v] = nm[v];
-      cp[2][v] = np[v];
-      cmm[v] = nmm[v];
-      cpp[v] = npp[v];
+    for (int w = 0; w < W; ++w)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) ld(p0 - 2 + i, w + 2, r[w][i]);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      ld(p0 - 1 + i, 1, rs[0][i]);
+      ld(p0 - 1 + i, W + 2, rs[1][i]);
     }
-    C cfx[V];
+    C om2[W][V], om1[W][V];
 #pragma unroll
-    for (int v = 0; v < V; ++v) cfx[v] = cf[v];
-    if (x + 1 < x1) {  // next row's loads, in flight while this row is computed
-      ld(p + 3, 0, n0);
-      ld(p + 2, -1, nm);
-      ld(p + 2, 1, np);
-      ld(p + 1, -2, nmm);
-      ld(p + 1, 2, npp);
-      loadv<S, V>(cfp + static_cast<int64_t>(x + 1 - x0) * J * K, cf);
+    for (int w = 0; w < W; ++w) {
+      unpackv<S, V>(r[w][0], om2[w]);
+      unpackv<S, V>(r[w][1], om1[w]);
+      unpackv<S, V>(r[w][2], oa[w]);
+      unpackv<S, V>(r[w][3], ob[w]);
     }
-    C res[V];
 #pragma unroll
-    for (int v = 0; v < V; ++v) {
-      const C c = c0[2][v];
-      if (x == x0) {
-        L0[v] = lap(c, c0[3][v], c0[1][v], cp[1][v], cm[1][v]);
-        const C Lim = lap(c0[1][v], c, c0[0][v], cp[0][v], cm[0][v]);
-        flx_lo[v] = limit<C>(L0[v] - Lim, c - c0[1][v]);
-      }
-      const C Lip = lap(c0[3][v], c0[4][v], c, cp[2][v], cm[2][v]);
-      const C Ljp = lap(cp[1][v], cp[2][v], cp[0][v], cpp[v], c);
-      const C Ljm = lap(cm[1][v], cm[2][v], cm[0][v], c, cmm[v]);
-      const C flx_hi = limit<C>(Lip - L0[v], c0[3][v] - c);
-      const C fly_hi = limit<C>(Ljp - L0[v], cp[1][v] - c);
-      const C fly_lo = limit<C>(L0[v] - Ljm, c - cm[1][v]);
-      C dv = flx_hi - flx_lo[v];
-      dv = dv + fly_hi;
-      dv = dv - fly_lo;
-      res[v] = c - cfx[v] * dv;
-      // carry to row p+1
-      L0[v] = Lip;
-      flx_lo[v] = flx_hi;
+    for (int s2 = 0; s2 < 2; ++s2) {
+      unpackv<S, V>(rs[s2][0], sa[s2]);
+      unpackv<S, V>(rs[s2][1], sb[s2]);
     }
-    C o[V];
 #pragma unroll
-    for (int v = 0; v < V; ++v) o[v] = res[v];
-    if (halo_dst)
-      storev<S, V>(halo_dst + p * si + (y + 2) * sj + k0, o);
-    else
-      storev<S, V>(out + (static_cast<int64_t>(x) * J + y) * K + k0, o);
-    // slide the window down one row
+    for (int w = 0; w < W; ++w) {
 #pragma unroll
-    for (int v = 0; v < V; ++v) {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) c0[r][v] = c0[r + 1][v];
-#pragma unroll
-      for (int r = 0; r < 2; ++r) {
-        cm[r][v] = cm[r + 1][v];
-        cp[r][v] = cp[r + 1][v];
+      for (int v = 0; v < V; ++v) {
+        const C c = oa[w][v];
+        const C rp = w + 1 < W ? oa[w + 1][v] : sb[1][v];   // (p, q+1)
+        const C lp = w > 0 ? oa[w - 1][v] : sb[0][v];       // (p, q-1)
+        const C rm = w + 1 < W ? om1[w + 1][v] : sa[1][v];  // (p-1, q+1)
+        const C lm = w > 0 ? om1[w - 1][v] : sa[0][v];      // (p-1, q-1)
+        L0[w][v] = lap(c, ob[w][v], om1[w][v], rp, lp);
+        const C Lim = lap(om1[w][v], c, om2[w][v], rm, lm);
+        fxl[w][v] = limit<C>(L0[w][v] - Lim, c - om1[w][v]);
       }
     }
   }
+  // the loads that complete row p0's window (and start each next row)
+  Raw nown[W], nsd[2], nfar[2], ncf[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) ld(p0 + 2, w + 2, nown[w]);
+  ld(p0 + 1, 1, nsd[0]);
+  ld(p0 + 1, W + 2, nsd[1]);
+  ld(p0, 0, nfar[0]);
+  ld(p0, W + 3, nfar[1]);
+  const S* __restrict__ cfp = coeff + (static_cast<int64_t>(x0) * J + y0) * K + k0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) loadraw<S, V>(cfp + w * K, ncf[w]);
+
+  // One output row x (in-field row p = x + 2): o0/o1/o2 hold the own rows
+  // p, p+1, p+2 (o2 filled here), s0/s1/s2 the side rows p-1, p, p+1 (s2
+  // filled here).
+  auto row = [&](int x, Own& o0, Own& o1, Own& o2, Side& s0, Side& s1, Side& s2) {
+    const int p = x + 2;
+    C cf[W][V];
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      unpackv<S, V>(nown[w], o2[w]);
+      unpackv<S, V>(ncf[w], cf[w]);
+    }
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+      unpackv<S, V>(nsd[f], s2[f]);
+      unpackv<S, V>(nfar[f], far[f]);
+    }
+    {  // next row's loads, in flight while this row is computed; after the
+       // field's last row they reload rows of the field (rows clamped), so
+       // they need no branch
+      const int pn = min(p + 1, I + 1);
+#pragma unroll
+      for (int w = 0; w < W; ++w) ld(pn + 2, w + 2, nown[w]);
+      ld(pn + 1, 1, nsd[0]);
+      ld(pn + 1, W + 2, nsd[1]);
+      ld(pn, 0, nfar[0]);
+      ld(pn, W + 3, nfar[1]);
+      const S* nc = cfp + static_cast<int64_t>(pn - 2 - x0) * J * K;
+#pragma unroll
+      for (int w = 0; w < W; ++w) loadraw<S, V>(nc + w * K, ncf[w]);
+    }
+    C res[W][V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      // Laplacians at row p of columns -1 .. W (own: carried), row p's values
+      C lc[W + 2], cv[W + 2];
+      lc[0] = lap(s1[0][v], s2[0][v], s0[0][v], o0[0][v], far[0][v]);
+      lc[W + 1] = lap(s1[1][v], s2[1][v], s0[1][v], far[1][v], o0[W - 1][v]);
+      cv[0] = s1[0][v];
+      cv[W + 1] = s1[1][v];
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        lc[w + 1] = L0[w][v];
+        cv[w + 1] = o0[w][v];
+      }
+      // j-fluxes across the faces (c-1 | c), c = 0 .. W (own column w sits
+      // at c = w + 1)
+      C fy[W + 1];
+#pragma unroll
+      for (int c = 0; c <= W; ++c) fy[c] = limit<C>(lc[c + 1] - lc[c], cv[c + 1] - cv[c]);
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const C c = o0[w][v];
+        const C rp1 = w + 1 < W ? o1[w + 1][v] : s2[1][v];  // (p+1, q+1)
+        const C lp1 = w > 0 ? o1[w - 1][v] : s2[0][v];      // (p+1, q-1)
+        const C Lip = lap(o1[w][v], o2[w][v], c, rp1, lp1);
+        const C flx_hi = limit<C>(Lip - L0[w][v], o1[w][v] - c);
+        C dv = flx_hi - fxl[w][v];
+        dv = dv + fy[w + 1];
+        dv = dv - fy[w];
+        res[w][v] = c - cf[w][v] * dv;
+        L0[w][v] = Lip;
+        fxl[w][v] = flx_hi;
+      }
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      if (y0 + w < yn) continue;  // stored by the group to the left
+      if (halo_dst)
+        storev<S, V>(halo_dst + p * si + static_cast<int64_t>(y0 + w + 2) * K + k0, res[w]);
+      else
+        storev<S, V>(out + (static_cast<int64_t>(x) * J + y0 + w) * K + k0, res[w]);
+    }
+  };
+  // Fully unrolled: the slots rotate and the carried values change
+  // registers at compile time, with no loop-back copies.
+#pragma unroll
+  for (int r = 0; r < STRIP; ++r) {
+    if (x0 + r >= x1) break;
+    if (r % 3 == 0)
+      row(x0 + r, oa, ob, oc, sa, sb, sc);
+    else if (r % 3 == 1)
+      row(x0 + r, ob, oc, oa, sb, sc, sa);
+    else
+      row(x0 + r, oc, oa, ob, sc, sa, sb);
+  }
 }
+
+// Marching shapes (tuning knob "hdiff.shape"): {W columns per lane, STRIP
+// rows per thread, VB bytes of k per lane}; a shape whose vector is
+// narrower than one element (4 bytes of FP64) runs the storage type's
+// default.
+struct MarchShape {
+  int w, strip, vb;
+};
+constexpr MarchShape kMarchShapes[] = {{0, 0, 0},   {1, 12, 16}, {1, 6, 16}, {2, 6, 16},
+                                       {2, 12, 16}, {1, 18, 16}, {4, 6, 8},  {2, 6, 8},
+                                       {4, 12, 8},  {4, 6, 4},   {4, 12, 4}, {2, 12, 8},
+                                       {2, 12, 4},  {3, 6, 8}};
+constexpr int kMarchShapeCount = sizeof(kMarchShapes) / sizeof(kMarchShapes[0]);
+std::atomic<int> g_hdiff_shape{0};
+
+// Default shape per storage type (knob value 0), the fastest at 256 x 256 x
+// 160 on the MI355X: FP64 and FP32 2 columns x 6 rows with 8-byte vectors,
+// BF16 4 x 6 with 4-byte vectors (two elements: the four columns' window
+// then fits in 110 VGPRs, four waves per SIMD).
+template <class S>
+constexpr int default_march_shape() { return sizeof(S) == 2 ? 9 : 7; }
 
 // ---------------------------------------------------------------- fused steps
 // hdiff_tb<S, TB>: up to TB iterated steps in one HBM round trip.  A
@@ -386,15 +475,42 @@ constexpr int hdiff_tb_max() { return sizeof(S) == 2 ? 2 : 4; }  // BF16 x4 exce
 template <class S>
 int run_hdiff(const S* in, S* out, const S* coeff, int I, int J, int K, int64_t nsteps,
               hipStream_t stream) {
-  constexpr int V = march_vec<S>();
-  const bool vec = K % V == 0 && aligned16(in) && aligned16(out) && aligned16(coeff);
-  const int xblocks = static_cast<int>(cdiv(static_cast<int64_t>(J) * (K / V), 256));
-  const int64_t mblocks = int64_t(xblocks) * cdiv(I, march_strip<S>());
-  NPBX_REQUIRE(!vec || mblocks < (int64_t(1) << 31), "hdiff: grid too large");
+  int shape = g_hdiff_shape.load();
+  if (shape <= 0 || shape >= kMarchShapeCount || kMarchShapes[shape].vb < int(sizeof(S)))
+    shape = default_march_shape<S>();
+  if (kMarchShapes[shape].w > J) shape = 1;  // a group needs J >= W columns
+  const int W = kMarchShapes[shape].w, strip = kMarchShapes[shape].strip;
+  const int V = kMarchShapes[shape].vb / static_cast<int>(sizeof(S));
+  const bool aligned = aligned16(in) && aligned16(out) && aligned16(coeff);
+  const bool vec = K % V == 0 && aligned;
+  const int64_t lanes = cdiv(J, W) * (K / V);
+  const int xblocks = static_cast<int>(cdiv(lanes, 256));
+  const int64_t mblocks = int64_t(xblocks) * cdiv(I, strip);
+  NPBX_REQUIRE(!vec || (lanes < (int64_t(1) << 31) && mblocks < (int64_t(1) << 31)),
+               "hdiff: grid too large");
   auto launch = [&](const S* src, S* next) -> int {
     if (vec) {
-      hipLaunchKernelGGL(hdiff_march<S>, dim3(static_cast<unsigned>(mblocks)), dim3(256), 0,
-                         stream, src, out, coeff, I, J, K, xblocks, next);
+      const dim3 grid(static_cast<unsigned>(mblocks));
+      auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, grid, dim3(256), 0, stream, src, out, coeff, I, J, K, xblocks,
+                           next);
+      };
+      constexpr int b4 = sizeof(S) <= 4 ? 4 : 8;  // FP64 has no 4-byte vector
+      switch (shape) {
+        case 1: go(hdiff_march<S, 1, 12, 16>); break;
+        case 2: go(hdiff_march<S, 1, 6, 16>); break;
+        case 3: go(hdiff_march<S, 2, 6, 16>); break;
+        case 4: go(hdiff_march<S, 2, 12, 16>); break;
+        case 5: go(hdiff_march<S, 1, 18, 16>); break;
+        case 6: go(hdiff_march<S, 4, 6, 8>); break;
+        case 7: go(hdiff_march<S, 2, 6, 8>); break;
+        case 8: go(hdiff_march<S, 4, 12, 8>); break;
+        case 9: go(hdiff_march<S, 4, 6, b4>); break;
+        case 10: go(hdiff_march<S, 4, 12, b4>); break;
+        case 11: go(hdiff_march<S, 2, 12, 8>); break;
+        case 12: go(hdiff_march<S, 2, 12, b4>); break;
+        default: go(hdiff_march<S, 3, 6, 8>); break;
+      }
       NPBX_LAUNCHED("hdiff_march");
     } else {
       const int64_t si = static_cast<int64_t>(J + 4) * K, sj = K;
@@ -422,7 +538,7 @@ int run_hdiff(const S* in, S* out, const S* coeff, int I, int J, int K, int64_t 
     NPBX_LAUNCHED("hdiff_copy_halo");
   }
   const S* src = in;
-  if (tb >= 2 && vec) {
+  if (tb >= 2 && aligned && K % Vec16<S>::N == 0) {  // whole 16-byte vectors per column
     constexpr int V = Vec16<S>::N;
     const int tiles_x = static_cast<int>(cdiv(I, kTbB)), tiles_y = static_cast<int>(cdiv(J, kTbB));
     const int64_t nb = int64_t(tiles_x) * tiles_y * (K / V);
@@ -461,6 +577,9 @@ int run_hdiff(const S* in, S* out, const S* coeff, int I, int J, int K, int64_t 
 
 int hdiff_tblock() { return g_hdiff_tblock.load(); }
 void set_hdiff_tblock(int tb) { g_hdiff_tblock.store(tb); }
+int hdiff_shape() { return g_hdiff_shape.load(); }
+void set_hdiff_shape(int shape) { g_hdiff_shape.store(shape); }
+int hdiff_shape_count() { return kMarchShapeCount; }
 
 }  // namespace npbx
 

@@ -1446,6 +1446,12 @@ extern "C" int npbx_set_tuning(const char* key, int64_t value) {
     set_jacobi_deep_depth(static_cast<int>(value));
     return NPBX_OK;
   }
+  if (std::strcmp(key, "hdiff.shape") == 0) {
+    NPBX_REQUIRE(value >= 0 && value < hdiff_shape_count(),
+                 "set_tuning: hdiff.shape must be in [0, %d]", hdiff_shape_count() - 1);
+    set_hdiff_shape(static_cast<int>(value));
+    return NPBX_OK;
+  }
   if (std::strcmp(key, "jacobi.deep_waves") == 0) {
     NPBX_REQUIRE(value == 0 || value == 4 || value == 8 || value == 16,
                  "set_tuning: jacobi.deep_waves must be 0 (by shape), 4, 8 or 16");
@@ -1465,7 +1471,7 @@ extern "C" int npbx_set_tuning(const char* key, int64_t value) {
   }
   return fail(NPBX_EINVAL,
               "set_tuning: unknown key '%s' (heat.rows, heat.seg, sweep.shape, hdiff.tblock, "
-              "adi.path, vadv.path, seidel.blocks, jacobi.deep, jacobi.deep_waves)",
+              "hdiff.shape, adi.path, vadv.path, seidel.blocks, jacobi.deep, jacobi.deep_waves)",
               key);
 }
 
@@ -1493,6 +1499,10 @@ extern "C" int npbx_get_tuning(const char* key, int64_t* value) {
   }
   if (std::strcmp(key, "jacobi.deep") == 0) {
     *value = jacobi_deep_depth();
+    return NPBX_OK;
+  }
+  if (std::strcmp(key, "hdiff.shape") == 0) {
+    *value = hdiff_shape();
     return NPBX_OK;
   }
   if (std::strcmp(key, "jacobi.deep_waves") == 0) {

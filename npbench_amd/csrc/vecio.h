@@ -25,16 +25,28 @@ __device__ __forceinline__ void load16(const S* __restrict__ p,
   for (int v = 0; v < V; ++v) out[v] = Storage<S>::load(tmp[v]);
 }
 
+// n storage elements of `in` packed into 32-bit words (BF16: a conversion
+// instruction per pair).
+template <class S, int N>
+__device__ __forceinline__ void pack_words(const typename Storage<S>::C (&in)[N],
+                                           uint32_t (&w)[N * sizeof(S) / 4]) {
+  if constexpr (sizeof(S) == 2) {
+#pragma unroll
+    for (int i = 0; i < N / 2; ++i) w[i] = f32x2_to_bf16x2_bits(in[2 * i], in[2 * i + 1]);
+  } else {
+    S tmp[N];
+#pragma unroll
+    for (int v = 0; v < N; ++v) tmp[v] = Storage<S>::store(in[v]);
+    __builtin_memcpy(w, tmp, N * sizeof(S));
+  }
+}
+
 template <class S>
 __device__ __forceinline__ void store16(S* __restrict__ p,
                                         const typename Storage<S>::C (&in)[Vec16<S>::N]) {
-  constexpr int V = Vec16<S>::N;
-  S tmp[V];
-#pragma unroll
-  for (int v = 0; v < V; ++v) tmp[v] = Storage<S>::store(in[v]);
-  uint4 raw;
-  __builtin_memcpy(&raw, tmp, 16);
-  *reinterpret_cast<uint4*>(p) = raw;
+  uint32_t w[4];
+  pack_words<S, Vec16<S>::N>(in, w);
+  *reinterpret_cast<uint4*>(p) = make_uint4(w[0], w[1], w[2], w[3]);
 }
 
 // V consecutive elements in one 8- or 16-byte access (V * sizeof(S)).
@@ -52,19 +64,54 @@ __device__ __forceinline__ void loadv(const S* __restrict__ p, typename Storage<
   }
 }
 
+// V consecutive storage elements as raw 32-bit words, loaded in one 8- or
+// 16-byte access (loadraw) and converted to the arithmetic type later
+// (unpackv): a prefetched BF16 vector holds half the registers of its
+// converted form, and the conversion does not wait on the load early.
+template <class S, int V>
+struct RawVec {
+  static constexpr int kWords = V * static_cast<int>(sizeof(S)) / 4;
+  uint32_t w[kWords];
+};
+
+template <class S, int V>
+__device__ __forceinline__ void loadraw(const S* __restrict__ p, RawVec<S, V>& r) {
+  static_assert(V * sizeof(S) == 16 || V * sizeof(S) == 8 || V * sizeof(S) == 4,
+                "4-, 8- or 16-byte vectors");
+  if constexpr (V * sizeof(S) == 16) {
+    const uint4 x = *reinterpret_cast<const uint4*>(p);
+    r.w[0] = x.x, r.w[1] = x.y, r.w[2] = x.z, r.w[3] = x.w;
+  } else if constexpr (V * sizeof(S) == 8) {
+    const uint2 x = *reinterpret_cast<const uint2*>(p);
+    r.w[0] = x.x, r.w[1] = x.y;
+  } else {
+    r.w[0] = *reinterpret_cast<const uint32_t*>(p);
+  }
+}
+
+template <class S, int V>
+__device__ __forceinline__ void unpackv(const RawVec<S, V>& r, typename Storage<S>::C (&out)[V]) {
+  S tmp[V];
+  __builtin_memcpy(tmp, r.w, sizeof(tmp));
+#pragma unroll
+  for (int v = 0; v < V; ++v) out[v] = Storage<S>::load(tmp[v]);
+}
+
 template <class S, int V>
 __device__ __forceinline__ void storev(S* __restrict__ p,
                                        const typename Storage<S>::C (&in)[V]) {
-  static_assert(V * sizeof(S) == 16 || V * sizeof(S) == 8, "8- or 16-byte vectors");
+  static_assert(V * sizeof(S) == 16 || V * sizeof(S) == 8 || V * sizeof(S) == 4,
+                "4-, 8- or 16-byte vectors");
   if constexpr (V * sizeof(S) == 16) {
     store16(p, in);
+  } else if constexpr (V * sizeof(S) == 8) {
+    uint32_t w[2];
+    pack_words<S, V>(in, w);
+    *reinterpret_cast<uint2*>(p) = make_uint2(w[0], w[1]);
   } else {
-    S tmp[V];
-#pragma unroll
-    for (int v = 0; v < V; ++v) tmp[v] = Storage<S>::store(in[v]);
-    uint2 raw;
-    __builtin_memcpy(&raw, tmp, 8);
-    *reinterpret_cast<uint2*>(p) = raw;
+    uint32_t w[1];
+    pack_words<S, V>(in, w);
+    *reinterpret_cast<uint32_t*>(p) = w[0];
   }
 }
 

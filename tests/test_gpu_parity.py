@@ -197,6 +197,55 @@ def test_hdiff_reduced_precision(st, precision, kernel_tol, f64_tol):
         assert err_f64 <= f64_tol
 
 
+@pytest.fixture
+def hdiff_shape():
+    """Sets the library's hdiff.shape knob for one test and restores it."""
+    from npbench_amd import _native
+    old = _native.get_tuning("hdiff.shape")
+    yield lambda v: _native.set_tuning("hdiff.shape", v)
+    _native.set_tuning("hdiff.shape", old)
+
+
+def _unaligned(a, dt):
+    """A device copy of `a` whose data starts one element past a 16-byte
+    boundary, which sends hdiff to its one-point-per-thread kernel."""
+    buf = torch.empty(a.size + 1, dtype=dt, device="cuda")
+    v = buf[1:].view(a.shape)
+    v.copy_(torch.from_numpy(a).to(dt))
+    return v
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16"])
+@pytest.mark.parametrize("I,J,K", [(1, 1, 8), (5, 7, 16), (17, 33, 40), (9, 3, 24),
+                                   (64, 64, 160)])
+@pytest.mark.parametrize("nsteps", [1, 3])
+def test_hdiff_march_shapes_agree(st, hdiff_shape, precision, I, J, K, nsteps):
+    """Every marching shape (hdiff.shape 1 .. 13: 1 to 4 columns per lane,
+    6 to 18 rows per thread, 4- to 16-byte k-vectors) gives
+    bitwise the one-point-per-thread kernel's result, for odd J (the last
+    column group shifted left), J below the group width, one-row strips and
+    iterated steps; FP64 also bitwise the oracle's."""
+    dt = {"fp64": torch.float64, "fp32": torch.float32, "bf16": torch.bfloat16}[precision]
+    inf, out, coeff = _hdiff_inputs(I, J, K, I + 3 * J + K + nsteps)
+    d_in, d_c = dev(inf, dt), dev(coeff, dt)
+    ref_out = _unaligned(out, dt)
+    st.hdiff(_unaligned(inf, dt), ref_out, _unaligned(coeff, dt), nsteps=nsteps)
+    want = ref_out.contiguous()
+    if precision == "fp64":
+        r = out.copy()
+        ref.hdiff_iterated(inf, r, coeff, nsteps) if nsteps > 1 else ref.hdiff(inf, r, coeff)
+        assert_bitwise(host(want), r, "point kernel vs oracle")
+    for shape in range(1, 14):
+        print(f"hdiff shape={shape} {precision} {I}x{J}x{K} nsteps={nsteps}", flush=True)
+        hdiff_shape(shape)
+        o = dev(out, dt)
+        st.hdiff(d_in, o, d_c, nsteps=nsteps)
+        torch.cuda.synchronize()
+        same = torch.equal(o.view(torch.int16), want.view(torch.int16)) \
+            if dt == torch.bfloat16 else torch.equal(o, want)
+        assert same, f"shape {shape}"
+
+
 # ---------------------------------------------------------------- vadv
 def _vadv_inputs(I, J, K, seed):
     rng = np.random.default_rng(seed)
@@ -329,6 +378,25 @@ def test_cast_bf16_rne(st):
     b32 = st.cast(dev(x32), torch.bfloat16)
     want = torch.from_numpy(x32).to(torch.bfloat16)
     assert torch.equal(b32.cpu().view(torch.int16), want.view(torch.int16))
+
+
+def test_bf16_rounding_of_every_float(st):
+    """All 2^32 FP32 bit patterns through npbx_cast, which rounds as every
+    BF16 store of the library does (the conversion instruction): bitwise
+    torch's round-to-nearest-even for every non-NaN pattern (denormals,
+    ties, overflow to infinity included) and a NaN for every NaN."""
+    chunk = 1 << 28
+    base = torch.arange(chunk, dtype=torch.int32, device="cuda")
+    for c in range(16):
+        print(f"bf16 rounding: patterns {c << 28:#x} ..", flush=True)
+        bits = base + ((c << 28) - (1 << 32 if c >= 8 else 0))
+        x = bits.view(torch.float32)
+        got = st.cast(x, torch.bfloat16).view(torch.int16)
+        want = x.to(torch.bfloat16).view(torch.int16)
+        nan = (bits & 0x7FFFFFFF) > 0x7F800000
+        assert torch.equal(got[~nan], want[~nan]), f"chunk {c}"
+        assert bool(((got[nan].to(torch.int32) & 0x7FFF) > 0x7F80).all()), f"chunk {c}: NaN"
+        del bits, x, got, want, nan
 
 
 @pytest.fixture
