@@ -842,7 +842,7 @@ def main():
                          "(profiler runs)")
     ap.add_argument("--vadv-path", type=int, default=0,
                     help="with --only vadv: the vadv.path knob (0 by shape, 1 workspace, "
-                         "2 on-chip)")
+                         "2 c' and d' in LDS, 3 c' in registers)")
     ap.add_argument("--only", choices=["headline", "fp64", "fp32", "copy", "kernels", "serve",
                                        "vadv"],
                     help="run one part and print its figures (for profiler runs)")

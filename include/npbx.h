@@ -224,7 +224,12 @@ int npbx_pack_cols(int dtype, const void *A, const void *B, int64_t n,
  * the default: by size; 1: all time steps in one single-workgroup launch, up
  * to 512 lines; 2: one launch per half step), "vadv.path" = vadv kernel (0,
  * the default: by shape; 1: c' and d' through a device workspace; 2: c' and
- * d' in LDS where K <= 160 and columns hold whole 16-byte vectors),
+ * d' in LDS where K <= 160 and columns hold whole 16-byte vectors; 3: c' in
+ * registers, d' in LDS, 80 < K <= 160; 4: loads coalesced and staged in
+ * LDS, c' and d' in registers, 80 < K <= 160 with whole 64-byte chunks of
+ * levels -- the default where it applies), "hdiff.shape" = hdiff marching
+ * shape (0, the default: by storage type; 1 .. 13: columns per lane, rows
+ * per thread, bytes of k per lane),
  * "seidel.blocks" = wavefront
  * slot-blocks (slots x 256-column blocks) per seidel_2d launch (default 2048,
  * about 40 ms), "jacobi.deep" = 2-D Jacobi sweeps per launch at tblock <= 1

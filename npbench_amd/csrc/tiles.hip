@@ -1459,7 +1459,7 @@ extern "C" int npbx_set_tuning(const char* key, int64_t value) {
     return NPBX_OK;
   }
   if (std::strcmp(key, "vadv.path") == 0) {
-    NPBX_REQUIRE(value >= 0 && value <= 2, "set_tuning: vadv.path must be 0, 1 or 2");
+    NPBX_REQUIRE(value >= 0 && value <= 4, "set_tuning: vadv.path must be in [0, 4]");
     set_vadv_path(static_cast<int>(value));
     return NPBX_OK;
   }

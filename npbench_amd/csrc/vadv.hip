@@ -25,6 +25,7 @@
 #include <type_traits>
 
 #include "common.h"
+#include "division.h"
 #include "vecio.h"
 
 namespace npbx {
@@ -265,6 +266,38 @@ __global__ __launch_bounds__(kWave) void vadv_wave(S* __restrict__ utens_stage,
 // and middle chunks run in pairs of straight-line code with no branch
 // between levels.  The arithmetic is vadv_wave's, operation for operation
 // (both waves form a_k, c_k and the coefficients from the same sums).
+// 1 / den with the bits of the division operator: the compiler's IEEE
+// division sequence restated without its scaling steps (division_reciprocal,
+// then r = fma(-den, y, 1), q = fma(r, y, y)) where that sequence does not
+// rescale (den in range and 1/den below 2^765); a lane outside that range
+// takes the division operator, behind a wave-uniform branch so that the
+// common path does not also compute the full sequence.
+__device__ __forceinline__ double pivot_reciprocal(double den) {
+  const double y = division_reciprocal(den);
+  const double r = __builtin_fma(-den, y, 1.0);
+  double q = __builtin_fma(r, y, y);
+  const bool ok = den_in_range(den) && fabs(y) < 0x1p765;
+  if (__builtin_amdgcn_ballot_w64(!ok) != 0) {
+    double d = den;
+    asm volatile("" : "+v"(d));  // keeps the division inside the branch
+    if (!ok) q = 1.0 / d;
+  }
+  return q;
+}
+
+// The short form alone: 1 / den where `ok` stays true, else a value to be
+// replaced (the caller redoes its chunk with the division operator).
+__device__ __forceinline__ double short_reciprocal(double den, bool& ok) {
+  const double y = division_reciprocal(den);
+  const double r = __builtin_fma(-den, y, 1.0);
+  ok = ok && den_in_range(den) && fabs(y) < 0x1p765;
+  return __builtin_fma(r, y, y);
+}
+
+// A 16-byte register quad (a vector type, so arrays of them stay in
+// registers).
+typedef uint32_t q32x4 __attribute__((ext_vector_type(4)));
+
 template <class S>
 struct PairShape {
   static constexpr int L = 64 / static_cast<int>(sizeof(S));  // levels per chunk
@@ -558,12 +591,549 @@ __global__ __launch_bounds__(2 * kWave) void vadv_pair(S* __restrict__ utens_sta
   if (m == 0) back(Gn{}, 0, pb);
 }
 
+// ---- c' in registers (80 < K <= 160) ---------------------------------------
+// vadv_pair's split with wave 0 keeping its columns' c'_k in registers (a
+// fully unrolled march over the NLEV levels: 2 VGPRs per level), so LDS
+// holds only inv_k / d'_k: 80 KiB for 64 columns x 160 levels, two
+// workgroups per CU where vadv_pair fits one.  Wave 0 also runs the back
+// substitution (c' from its registers, d' from LDS, u_pos read again).  The
+// pivot's reciprocal is the compiler's IEEE division sequence restated
+// without its scaling steps (division_reciprocal, then r = fma(-den, y, 1),
+// inv = fma(r, y, y)), used where that sequence does not rescale (den
+// in range and 1/den below 2^765); elsewhere the division operator.  The
+// arithmetic is vadv_pair's, operation for operation.
+template <class S, int NLEV>
+__global__ __launch_bounds__(2 * kWave) void vadv_reg(S* __restrict__ utens_stage,
+                                                      const S* __restrict__ u_stage,
+                                                      const S* __restrict__ wcon,
+                                                      const S* __restrict__ u_pos,
+                                                      const S* __restrict__ utens, double dtr,
+                                                      int J, int K, int64_t ncols) {
+  constexpr int L = PairShape<S>::L, V = PairShape<S>::V;
+  constexpr int NCH = NLEV / L;  // chunks the unrolled march can hold
+  static_assert(NLEV % L == 0, "whole chunks of levels");
+  __shared__ double dp[NLEV * kWave];  // inv_k from wave 0, then d'_k from wave 1
+  const int lane = threadIdx.x % kWave;
+  const int64_t c0 = static_cast<int64_t>(xcd_tile(blockIdx.x, gridDim.x)) * kWave;
+  const int64_t col = c0 + lane;
+  const bool active = col < ncols;
+  const int64_t cc = active ? col : c0;
+  const int nchunks = (K + L - 1) / L;
+
+  auto load_streams = [&](const S* const* src, auto ns, int n, S (&r)[decltype(ns)::value][L]) {
+#pragma unroll
+    for (int q = 0; q < decltype(ns)::value; ++q) {
+#pragma unroll
+      for (int m = 0; m < L / V; ++m) {
+        const int k = min(max(n * L + m * V, 0), K - V);
+        const uint4 raw = *reinterpret_cast<const uint4*>(src[q] + k);
+        __builtin_memcpy(&r[q][m * V], &raw, 16);
+      }
+    }
+  };
+  // 1 / den: the division sequence's bits without its scaling steps where
+  // they do not apply
+
+  if (threadIdx.x < kWave) {
+    // ======== wave 0: the pivot chain, c' kept in registers (every loop
+    // over chunks and levels fully unrolled, guards instead of exits)
+    using N2 = std::integral_constant<int, 2>;
+    const S* src[2] = {wcon + cc * K, wcon + (cc + J) * K};
+    double cpr[NLEV];
+    S ra[2][L], rb[2][L];
+    load_streams(src, N2{}, 0, ra);
+    load_streams(src, N2{}, 1, rb);
+    double s_cur = 0, c_prev = 0;
+#pragma unroll
+    for (int n = 0; n < NCH; ++n) {
+      if (n < nchunks) {
+        double sw[L];
+        auto& cur = (n & 1) ? rb : ra;
+#pragma unroll
+        for (int l = 0; l < L; ++l)
+          sw[l] = static_cast<double>(cur[1][l]) + static_cast<double>(cur[0][l]);
+        load_streams(src, N2{}, n + 2, cur);  // two chunks ahead (clamped past the end)
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+          const int k = n * L + l;
+          if (k == 0) {
+            s_cur = sw[0];
+          } else if (k == 1) {  // level 0 (vadv_numpy.py:15-30)
+            const double gcv = 0.25 * sw[1];
+            const double ck = gcv * kBetP;
+            const double bcol = dtr - ck;
+            const double inv = pivot_reciprocal(bcol);
+            c_prev = ck * inv;
+            cpr[0] = c_prev;
+            dp[lane] = inv;
+            s_cur = sw[1];
+          } else if (k < K) {  // level k-1 (vadv_numpy.py:32-53)
+            const double gav = -0.25 * s_cur;
+            const double gcv = 0.25 * sw[l];
+            const double acol = gav * kBetP;
+            const double ck = gcv * kBetP;
+            const double bcol = dtr - acol - ck;
+            const double inv = pivot_reciprocal(bcol - c_prev * acol);
+            c_prev = ck * inv;
+            cpr[k - 1] = c_prev;
+            dp[(k - 1) * kWave + lane] = inv;
+            s_cur = sw[l];
+          }
+        }
+        if (n == nchunks - 1) {  // level K-1 (vadv_numpy.py:55-68)
+          const double gav = -0.25 * s_cur;
+          const double acol = gav * kBetP;
+          const double bcol = dtr - acol;
+          dp[(K - 1) * kWave + lane] = pivot_reciprocal(bcol - c_prev * acol);
+        }
+        __syncthreads();
+      }
+    }
+    __syncthreads();  // wave 1's last chunk: every d'_k is in LDS
+
+    // ---- back substitution (vadv_numpy.py:70-78), top chunk first; u_pos
+    // of the chunk below is in flight while a chunk is solved
+    using N1 = std::integral_constant<int, 1>;
+    const S* up_src[1] = {u_pos + cc * K};
+    S* out = utens_stage + cc * K;
+    double x = dp[(K - 1) * kWave + lane];
+    S pa[1][L], pb[1][L];  // chunk n's u_pos in pb when n is odd, else in pa
+    if ((nchunks - 1) & 1)
+      load_streams(up_src, N1{}, nchunks - 1, pb);
+    else
+      load_streams(up_src, N1{}, nchunks - 1, pa);
+#pragma unroll
+    for (int n = NCH - 1; n >= 0; --n) {
+      if (n < nchunks) {
+        auto& cur = (n & 1) ? pb : pa;
+        auto& nxt = (n & 1) ? pa : pb;
+        if (n > 0) load_streams(up_src, N1{}, n - 1, nxt);
+        S o[L];
+#pragma unroll
+        for (int l = L - 1; l >= 0; --l) {
+          const int k = n * L + l;
+          if (k < K - 1) x = dp[k * kWave + lane] - cpr[k] * x;
+          o[l] = static_cast<S>(dtr * (x - static_cast<double>(cur[0][l])));
+        }
+        if (active) {
+#pragma unroll
+          for (int m = 0; m < L / V; ++m) {
+            if (n * L + m * V < K) {
+              uint4 raw;
+              __builtin_memcpy(&raw, &o[m * V], 16);
+              *reinterpret_cast<uint4*>(out + n * L + m * V) = raw;
+            }
+          }
+        }
+      }
+    }
+    return;
+  }
+
+  // ======== wave 1: right-hand sides and d' (vadv_pair's wave 1 forward
+  // pass), d'_{K-1} written to LDS as well
+  const S* src[6] = {utens_stage + cc * K, u_stage + cc * K, u_pos + cc * K,
+                     utens + cc * K,       wcon + cc * K,    wcon + (cc + J) * K};
+  using N6 = std::integral_constant<int, 6>;
+  double u_prev = 0, u_cur = 0, s_cur = 0, base_cur = 0, d_prev = 0;
+  S ra[6][L], rb[6][L];
+  load_streams(src, N6{}, 0, ra);
+  load_streams(src, N6{}, 1, rb);
+  for (int n = 0; n < nchunks; n += 2) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int nn = n + h;
+      if (nn >= nchunks) break;
+      auto& cur = h ? rb : ra;
+      double u[L], sw[L], base[L];
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        u[l] = static_cast<double>(cur[1][l]);
+        sw[l] = static_cast<double>(cur[5][l]) + static_cast<double>(cur[4][l]);
+        base[l] = (dtr * static_cast<double>(cur[2][l]) + static_cast<double>(cur[3][l])) +
+                  static_cast<double>(cur[0][l]);
+      }
+      load_streams(src, N6{}, nn + 2, cur);
+      __syncthreads();  // wave 0 has chunk nn's pivots
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        const int k = nn * L + l;
+        if (k >= K) break;
+        if (k == 0) {
+          u_cur = u[0];
+          s_cur = sw[0];
+          base_cur = base[0];
+          continue;
+        }
+        if (k == 1) {  // level 0 (vadv_numpy.py:15-30)
+          const double gcv = 0.25 * sw[1];
+          const double cs = gcv * kBetM;
+          const double corr = -cs * (u[1] - u_cur);
+          const double d = base_cur + corr;
+          d_prev = d * dp[lane];
+          dp[lane] = d_prev;
+        } else {  // level k-1
+          const double gav = -0.25 * s_cur;
+          const double gcv = 0.25 * sw[l];
+          const double as_ = gav * kBetM;
+          const double cs = gcv * kBetM;
+          const double acol = gav * kBetP;
+          const double corr = -as_ * (u_prev - u_cur) - cs * (u[l] - u_cur);
+          const double d = base_cur + corr;
+          double* slot = &dp[(k - 1) * kWave + lane];
+          d_prev = (d - d_prev * acol) * *slot;
+          *slot = d_prev;
+        }
+        u_prev = u_cur;
+        u_cur = u[l];
+        s_cur = sw[l];
+        base_cur = base[l];
+      }
+      if (nn == nchunks - 1) {  // level K-1 (vadv_numpy.py:55-68)
+        const double gav = -0.25 * s_cur;
+        const double as_ = gav * kBetM;
+        const double acol = gav * kBetP;
+        const double corr = -as_ * (u_prev - u_cur);
+        const double d = base_cur + corr;
+        double* slot = &dp[(K - 1) * kWave + lane];
+        *slot = (d - d_prev * acol) * *slot;
+      }
+    }
+  }
+  __syncthreads();  // d' complete
+}
+
+// ---- coalesced loads, c' and d' in registers (80 < K <= 160) ---------------
+// vadv_stage: the two-wave split of vadv_reg with every field chunk loaded
+// coalesced -- a wave load covers 16 columns' 64-byte chunk segments, four
+// lanes per segment, instead of one line per lane -- and turned to lane =
+// column through LDS (a column's four 16-byte quads at XOR-swizzled slots,
+// so both the coalesced writes and the per-column reads are free of bank
+// conflicts).  Wave 0 keeps c'_k in registers, wave 1 keeps d'_k in
+// registers, so LDS holds only the staged chunks and two small rings (inv_k
+// from wave 0 to wave 1 on the way down, c'_k on the way back up): 73 KiB
+// (FP32) or 65 KiB (FP64) per 64 columns, two workgroups per CU.  K is a
+// whole number of chunks (L = 64 bytes of levels), so a chunk's levels run
+// with no branch between them and its ring entries are read together.
+//   forward, interval n (n = 0 .. nchunks, a barrier after each):
+//     wave 0 stages wcon's chunk n+1 (T0, three buffers), loads chunk n+3,
+//       and runs the pivot chain of chunk n from T0, inv_k into ring[n % 2];
+//     wave 1 stages its four fields' chunk n (T1, two buffers), loads chunk
+//       n+2, and forms d'_k of chunk n-1 from T1, T0 and ring[(n-1) % 2]
+//   (loads two intervals ahead, in two register sets);
+//   backward, interval t (t = nchunks-1 .. -1):
+//     wave 0 writes c'_k of chunk t into ring[t % 2];
+//     wave 1 stages u_pos's chunk t, loads chunk t-1, and solves chunk t+1
+//       (x_k = d'_k - c'_k x_{k+1}, out = dtr (x - u_pos)), the output
+//       turned back to the coalesced layout through LDS for the stores.
+// Chunk loops are unrolled (register arrays need constant indices); the
+// arithmetic is vadv_reg's, operation for operation.
+template <class S>
+__global__ __launch_bounds__(2 * kWave) void vadv_stage(S* __restrict__ utens_stage,
+                                                        const S* __restrict__ u_stage,
+                                                        const S* __restrict__ wcon,
+                                                        const S* __restrict__ u_pos,
+                                                        const S* __restrict__ utens, double dtr,
+                                                        int J, int K, int64_t ncols) {
+  constexpr int NLEV = 160;
+  constexpr int L = 64 / static_cast<int>(sizeof(S));  // levels per chunk
+  constexpr int V = 16 / static_cast<int>(sizeof(S));  // levels per 16-byte quad
+  constexpr int NCH = NLEV / L;
+  __shared__ q32x4 t0[3][2][4 * kWave];  // wcon rows i, i+1: [buffer][stream][slot]
+  __shared__ q32x4 t1[2][4][4 * kWave];  // utens_stage, u_stage, u_pos, utens
+  __shared__ double ring[2][(L + 1) * kWave];
+  const int lane = threadIdx.x % kWave;
+  const int64_t c0 = static_cast<int64_t>(xcd_tile(blockIdx.x, gridDim.x)) * kWave;
+  const int nchunks = K / L;
+  // coalesced role: load m of a stream's chunk gives lane g quad g % 4 of
+  // column 16 m + g / 4 (columns past the last read the last column)
+  const int lq = lane & 3, lcol = lane >> 2;
+  const int sw_w = (lane >> 4) & 3;  // swizzle of the columns this lane writes
+  const int sw_r = (lane >> 2) & 3;  // swizzle of column `lane`
+  auto src_of = [&](const S* f, int m, int n) {
+    const int64_t col = min(c0 + 16 * m + lcol, ncols - 1);
+    return f + col * K + n * L + lq * V;
+  };
+  auto load4 = [&](const S* f, int n, q32x4 (&r)[4]) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) r[m] = *reinterpret_cast<const q32x4*>(src_of(f, m, n));
+  };
+  auto stage = [&](q32x4* buf, const q32x4 (&r)[4]) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) buf[(16 * m + lcol) * 4 + (lq ^ sw_w)] = r[m];
+  };
+  // quad j (V levels) of column `lane` of a staged chunk, as doubles
+  auto quad = [&](const q32x4* buf, int j, double (&out)[V]) {
+    const q32x4 q = buf[lane * 4 + (j ^ sw_r)];
+    S v[V];
+    __builtin_memcpy(v, &q, 16);
+#pragma unroll
+    for (int e = 0; e < V; ++e) out[e] = static_cast<double>(v[e]);
+  };
+
+  if (threadIdx.x < kWave) {
+    // ======== wave 0: wcon, the pivot chain, c' in registers
+    const S* w0 = wcon;
+    const S* w1 = wcon + static_cast<int64_t>(J) * K;
+    double cpr[NLEV];
+    q32x4 ra[2][4], rb[2][4];  // chunk c's wcon rows in set c % 2
+    load4(w0, 0, ra[0]);
+    load4(w1, 0, rb[0]);
+    stage(t0[0][0], ra[0]);
+    stage(t0[0][1], rb[0]);
+    load4(w0, 1, ra[1]);  // nchunks >= 2 (K > 80)
+    load4(w1, 1, rb[1]);
+    if (nchunks > 2) {
+      load4(w0, 2, ra[0]);
+      load4(w1, 2, rb[0]);
+    }
+    double s_cur = 0, c_prev = 0;
+#pragma unroll
+    for (int n = 0; n <= NCH; ++n) {
+      if (n <= nchunks) {
+        if (n + 1 < nchunks) {  // stage chunk n+1, load chunk n+3 into its set
+          stage(t0[(n + 1) % 3][0], ra[(n + 1) % 2]);
+          stage(t0[(n + 1) % 3][1], rb[(n + 1) % 2]);
+          if (n + 3 < nchunks) {
+            load4(w0, n + 3, ra[(n + 1) % 2]);
+            load4(w1, n + 3, rb[(n + 1) % 2]);
+          }
+        }
+        if (n < nchunks) {
+          // The chunk's levels with the short reciprocal; if any lane's
+          // pivot left its range, the chunk again from the same state with
+          // the division operator (bitwise the short form where it holds).
+          double* rg = ring[n % 2];
+          auto levels = [&](auto full) {
+            bool ok = true;
+            auto recip = [&](double den) {
+              if constexpr (decltype(full)::value)
+                return 1.0 / den;
+              else
+                return short_reciprocal(den, ok);
+            };
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {  // quad j: levels n L + j V ..
+              double sw[V];
+              {
+                double a[V];
+                quad(t0[n % 3][0], j, a);
+                quad(t0[n % 3][1], j, sw);
+#pragma unroll
+                for (int e = 0; e < V; ++e) sw[e] = sw[e] + a[e];
+              }
+#pragma unroll
+              for (int e = 0; e < V; ++e) {
+                const int l = j * V + e;
+                const int k = n * L + l;
+                if (k == 0) {
+                  s_cur = sw[e];
+                } else if (k == 1) {  // level 0 (vadv_numpy.py:15-30)
+                  const double gcv = 0.25 * sw[e];
+                  const double ck = gcv * kBetP;
+                  const double bcol = dtr - ck;
+                  const double inv = recip(bcol);
+                  c_prev = ck * inv;
+                  cpr[0] = c_prev;
+                  rg[l * kWave + lane] = inv;
+                  s_cur = sw[e];
+                } else {  // level k-1 (vadv_numpy.py:32-53)
+                  const double gav = -0.25 * s_cur;
+                  const double gcv = 0.25 * sw[e];
+                  const double acol = gav * kBetP;
+                  const double ck = gcv * kBetP;
+                  const double bcol = dtr - acol - ck;
+                  const double inv = recip(bcol - c_prev * acol);
+                  c_prev = ck * inv;
+                  cpr[k - 1] = c_prev;
+                  rg[l * kWave + lane] = inv;
+                  s_cur = sw[e];
+                }
+              }
+            }
+            if (n == nchunks - 1) {  // level K-1 (vadv_numpy.py:55-68)
+              const double gav = -0.25 * s_cur;
+              const double acol = gav * kBetP;
+              const double bcol = dtr - acol;
+              rg[L * kWave + lane] = recip(bcol - c_prev * acol);
+            }
+            return ok;
+          };
+          const double c_start = c_prev, s_start = s_cur;
+          if (__builtin_amdgcn_ballot_w64(!levels(std::false_type{})) != 0) {
+            c_prev = c_start;
+            s_cur = s_start;
+            levels(std::true_type{});
+          }
+        }
+        __syncthreads();
+      }
+    }
+    // ---- backward: c' of chunk t into ring[t % 2] (entries 0 .. L-1 hold
+    // levels t L .. t L + L - 1)
+#pragma unroll
+    for (int t = NCH - 1; t >= -1; --t) {
+      if (t <= nchunks - 1) {
+        if (t >= 0) {
+#pragma unroll
+          for (int l = 0; l < L; ++l)
+            if (t * L + l < NLEV - 1) ring[t % 2][l * kWave + lane] = cpr[t * L + l];
+        }
+        __syncthreads();
+      }
+    }
+    return;
+  }
+
+  // ======== wave 1: the fields, d' in registers, the back substitution
+  double dpr[NLEV];
+  q32x4 st[2][4][4];  // chunk c's fields in set c % 2
+  const S* fields[4] = {utens_stage, u_stage, u_pos, utens};
+#pragma unroll
+  for (int f = 0; f < 4; ++f) load4(fields[f], 0, st[0][f]);
+#pragma unroll
+  for (int f = 0; f < 4; ++f) load4(fields[f], 1, st[1][f]);
+  double u_prev = 0, u_cur = 0, s_cur = 0, base_cur = 0, d_prev = 0;
+#pragma unroll
+  for (int n = 0; n <= NCH; ++n) {
+    if (n <= nchunks) {
+      if (n < nchunks) {  // stage chunk n, load chunk n+2 into its set
+#pragma unroll
+        for (int f = 0; f < 4; ++f) stage(t1[n % 2][f], st[n % 2][f]);
+        if (n + 2 < nchunks) {
+#pragma unroll
+          for (int f = 0; f < 4; ++f) load4(fields[f], n + 2, st[n % 2][f]);
+        }
+      }
+      if (n >= 1) {  // d' of chunk p = n-1
+        const int p = n - 1;
+        const double* rg = ring[p % 2];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {  // quad j
+          double u[V], sw[V], base[V], inv[V];
+#pragma unroll
+          for (int e = 0; e < V; ++e) inv[e] = rg[(j * V + e) * kWave + lane];
+          {
+            double fs[V], fp[V], ft[V], a[V];
+            quad(t1[p % 2][0], j, fs);
+            quad(t1[p % 2][1], j, u);
+            quad(t1[p % 2][2], j, fp);
+            quad(t1[p % 2][3], j, ft);
+            quad(t0[p % 3][0], j, a);
+            quad(t0[p % 3][1], j, sw);
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+              sw[e] = sw[e] + a[e];
+              base[e] = (dtr * fp[e] + ft[e]) + fs[e];
+            }
+          }
+#pragma unroll
+          for (int e = 0; e < V; ++e) {
+            const int l = j * V + e;
+            const int k = p * L + l;
+            if (k == 0) {
+              u_cur = u[e];
+              s_cur = sw[e];
+              base_cur = base[e];
+              continue;
+            }
+            if (k == 1) {  // level 0 (vadv_numpy.py:15-30)
+              const double gcv = 0.25 * sw[e];
+              const double cs = gcv * kBetM;
+              const double corr = -cs * (u[e] - u_cur);
+              const double d = base_cur + corr;
+              d_prev = d * inv[e];
+              dpr[0] = d_prev;
+            } else {  // level k-1
+              const double gav = -0.25 * s_cur;
+              const double gcv = 0.25 * sw[e];
+              const double as_ = gav * kBetM;
+              const double cs = gcv * kBetM;
+              const double acol = gav * kBetP;
+              const double corr = -as_ * (u_prev - u_cur) - cs * (u[e] - u_cur);
+              const double d = base_cur + corr;
+              d_prev = (d - d_prev * acol) * inv[e];
+              dpr[k - 1] = d_prev;
+            }
+            u_prev = u_cur;
+            u_cur = u[e];
+            s_cur = sw[e];
+            base_cur = base[e];
+          }
+        }
+        if (p == nchunks - 1) {  // level K-1 (vadv_numpy.py:55-68)
+          const double gav = -0.25 * s_cur;
+          const double as_ = gav * kBetM;
+          const double acol = gav * kBetP;
+          const double corr = -as_ * (u_prev - u_cur);
+          const double d = base_cur + corr;
+          dpr[p * L + L - 1] = (d - d_prev * acol) * rg[L * kWave + lane];
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  // ---- backward (vadv_numpy.py:70-78): u_pos staged through T1, c' from
+  // the ring, the output through the wcon buffers (free now) for
+  // coalesced stores
+  q32x4 up4[4];
+  load4(u_pos, nchunks - 1, up4);
+  q32x4* obuf = t0[0][0];
+  double x = 0;
+#pragma unroll
+  for (int t = NCH - 1; t >= -1; --t) {
+    if (t <= nchunks - 1) {
+      if (t >= 0) {
+        stage(t1[t % 2][2], up4);
+        if (t >= 1) load4(u_pos, t - 1, up4);
+      }
+      const int q = t + 1;  // the chunk solved in this interval
+      if (q <= NCH - 1 && q < nchunks) {
+        const bool top = q == nchunks - 1;
+#pragma unroll
+        for (int j = 3; j >= 0; --j) {
+          double up[V], cpv[V];
+          quad(t1[q % 2][2], j, up);
+#pragma unroll
+          for (int e = 0; e < V; ++e) cpv[e] = ring[q % 2][(j * V + e) * kWave + lane];
+          S o[V];
+#pragma unroll
+          for (int e = V - 1; e >= 0; --e) {
+            const int l = j * V + e;
+            const int k = q * L + l;
+            if (l == L - 1)
+              x = top ? dpr[k] : dpr[k] - cpv[e] * x;
+            else
+              x = dpr[k] - cpv[e] * x;
+            o[e] = static_cast<S>(dtr * (x - up[e]));
+          }
+          q32x4 v;
+          __builtin_memcpy(&v, o, 16);
+          obuf[lane * 4 + (j ^ sw_r)] = v;
+        }
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          const int64_t col = c0 + 16 * m + lcol;
+          if (col < ncols)
+            *reinterpret_cast<q32x4*>(utens_stage + col * K + q * L + lq * V) =
+                obuf[(16 * m + lcol) * 4 + (lq ^ sw_w)];
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
 // Largest K the on-chip kernel holds, and its LDS levels for a given K.
 constexpr int kLdsMaxK = 160;
 inline int lds_levels(int64_t K) { return K <= 48 ? 48 : K <= 80 ? 80 : 160; }
 
-// Kernel choice ("vadv.path"): 0 = by shape, 1 = the workspace kernel, 2 =
-// the on-chip kernel where the shape allows it.
+// Kernel choice ("vadv.path"): 0 = by shape (vadv_stage where it applies,
+// else vadv_pair, else vadv_wave), 1 = vadv_wave (device workspace), 2 =
+// vadv_pair (c' and d' in LDS), 3 = vadv_reg (c' in registers), 4 =
+// vadv_stage (coalesced staged loads, c' and d' in registers); a kernel
+// that does not apply to the shape falls through to the next one.
 std::atomic<int> g_vadv_path{0};
 
 template <class S>
@@ -576,6 +1146,19 @@ int run_vadv(S* utens_stage, const S* u_stage, const S* wcon, const S* u_pos, co
                    aligned16(wcon) && aligned16(u_pos) && aligned16(utens);
   const dim3 grid(static_cast<unsigned>(cdiv(ncols, kWave)));
   const int path = g_vadv_path.load();
+  if (vec && K > 80 && K <= kLdsMaxK && path == 3) {
+    hipLaunchKernelGGL((vadv_reg<S, 160>), grid, dim3(2 * kWave), 0, stream, utens_stage, u_stage,
+                       wcon, u_pos, utens, dtr, static_cast<int>(J), static_cast<int>(K), ncols);
+    NPBX_LAUNCHED("vadv_reg");
+    return NPBX_OK;
+  }
+  // vadv_stage: 80 < K <= 160, whole 64-byte chunks of levels per column
+  if (vec && K > 80 && K <= kLdsMaxK && K % (64 / sizeof(S)) == 0 && (path == 4 || path == 0)) {
+    hipLaunchKernelGGL((vadv_stage<S>), grid, dim3(2 * kWave), 0, stream, utens_stage, u_stage,
+                       wcon, u_pos, utens, dtr, static_cast<int>(J), static_cast<int>(K), ncols);
+    NPBX_LAUNCHED("vadv_stage");
+    return NPBX_OK;
+  }
   if (vec && K <= kLdsMaxK && path != 1) {
     const int Ji = static_cast<int>(J), Ki = static_cast<int>(K);
     switch (lds_levels(K)) {

@@ -273,13 +273,49 @@ def test_vadv_fp64_bitwise(st, I, J, K):
     assert_bitwise(host(d[0]), r, "utens_stage")
 
 
+@pytest.fixture
+def vadv_path():
+    """Sets the library's vadv.path knob for one test and restores it."""
+    from npbench_amd import _native
+    old = _native.get_tuning("vadv.path")
+    yield lambda v: _native.set_tuning("vadv.path", v)
+    _native.set_tuning("vadv.path", old)
+
+
+@pytest.mark.parametrize("I,J,K", [(2, 64, 160), (3, 37, 84), (5, 13, 100), (4, 5, 158),
+                                   (1, 1, 96), (9, 30, 144)])
+def test_vadv_paths_agree(st, vadv_path, I, J, K):
+    """Every kernel (vadv.path 1: workspace, 2: c' and d' in LDS, 3: c' in
+    registers, 4: coalesced staged loads with c' and d' in registers; 3 and 4
+    for 80 < K <= 160) gives the oracle's bits with FP64 fields, and the
+    same bits as each other with FP32 fields."""
+    f = _vadv_inputs(I, J, K, 3 * I + J + K)
+    r = f[0].copy()
+    ref.vadv(r, *f[1:], 0.15)
+    got32 = []
+    for path in (1, 2, 3, 4):
+        print(f"vadv path={path} {I}x{J}x{K}", flush=True)
+        vadv_path(path)
+        d = [dev(a) for a in f]
+        st.vadv(*d, 0.15)
+        assert_bitwise(host(d[0]), r, f"utens_stage, path {path}")
+        d32 = [dev(a, torch.float32) for a in f]
+        st.vadv(*d32, 0.15)
+        torch.cuda.synchronize()
+        got32.append(d32[0])
+    assert all(torch.equal(got32[0], g) for g in got32[1:])
+
+
+@pytest.mark.parametrize("path", [0, 3, 4])
 @pytest.mark.parametrize("dtr,wscale", [(1e-250, 0.0), (1e-250, 1e-255), (1e250, 1.0),
-                                         (0.15, 1e-300)])
-def test_vadv_extreme_pivots(st, dtr, wscale):
+                                         (0.15, 1e-300), (1e-240, 0.0), (1e235, 1.0)])
+def test_vadv_extreme_pivots(st, vadv_path, path, dtr, wscale):
     """Pivots of magnitude about 1e-250 or 1e250 (tiny wcon leaves them at
-    dtr): bitwise the oracle at extreme exponents."""
+    dtr), where the register kernel's short reciprocal hands over to the
+    division operator: bitwise the oracle at extreme exponents."""
     I, J, K = 2, 64, 160
-    print(f"vadv extreme pivots dtr={dtr} wscale={wscale}", flush=True)
+    vadv_path(path)
+    print(f"vadv extreme pivots path={path} dtr={dtr} wscale={wscale}", flush=True)
     f = list(_vadv_inputs(I, J, K, 5))
     f[2] = wscale * f[2]
     r = f[0].copy()
