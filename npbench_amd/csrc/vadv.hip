@@ -285,13 +285,20 @@ __device__ __forceinline__ double pivot_reciprocal(double den) {
   return q;
 }
 
-// The short form alone: 1 / den where `ok` stays true, else a value to be
-// replaced (the caller redoes its chunk with the division operator).
-__device__ __forceinline__ double short_reciprocal(double den, bool& ok) {
+// The short form alone, keeping the smallest and largest |den| seen: where
+// 2^-764 <= |den| < 2^878 for every den of a chunk (reciprocal_range_ok),
+// the division sequence would not have rescaled (den's exponent within
+// [100, 1900], 1/den below 2^765 and normal), so the short form has its
+// bits; otherwise the caller redoes the chunk with the division operator.
+__device__ __forceinline__ double short_reciprocal(double den, double& lo, double& hi) {
   const double y = division_reciprocal(den);
   const double r = __builtin_fma(-den, y, 1.0);
-  ok = ok && den_in_range(den) && fabs(y) < 0x1p765;
+  lo = fmin(lo, fabs(den));
+  hi = fmax(hi, fabs(den));
   return __builtin_fma(r, y, y);
+}
+__device__ __forceinline__ bool reciprocal_range_ok(double lo, double hi) {
+  return lo >= 0x1p-764 && hi < 0x1p878;
 }
 
 // A 16-byte register quad (a vector type, so arrays of them stay in
@@ -905,12 +912,12 @@ __global__ __launch_bounds__(2 * kWave) void vadv_stage(S* __restrict__ utens_st
           // the division operator (bitwise the short form where it holds).
           double* rg = ring[n % 2];
           auto levels = [&](auto full) {
-            bool ok = true;
+            double lo = INFINITY, hi = 0.0;
             auto recip = [&](double den) {
               if constexpr (decltype(full)::value)
                 return 1.0 / den;
               else
-                return short_reciprocal(den, ok);
+                return short_reciprocal(den, lo, hi);
             };
 #pragma unroll
             for (int j = 0; j < 4; ++j) {  // quad j: levels n L + j V ..
@@ -957,7 +964,7 @@ __global__ __launch_bounds__(2 * kWave) void vadv_stage(S* __restrict__ utens_st
               const double bcol = dtr - acol;
               rg[L * kWave + lane] = recip(bcol - c_prev * acol);
             }
-            return ok;
+            return reciprocal_range_ok(lo, hi);
           };
           const double c_start = c_prev, s_start = s_cur;
           if (__builtin_amdgcn_ballot_w64(!levels(std::false_type{})) != 0) {
@@ -969,17 +976,36 @@ __global__ __launch_bounds__(2 * kWave) void vadv_stage(S* __restrict__ utens_st
         __syncthreads();
       }
     }
-    // ---- backward: c' of chunk t into ring[t % 2] (entries 0 .. L-1 hold
-    // levels t L .. t L + L - 1)
+    // ---- backward, interval t: c' of chunk t into ring[t % 2] (entries
+    // 0 .. L-1 hold levels t L ..), u_pos's chunk t staged (loaded one
+    // interval ahead), and chunk t+2's output (solved by wave 1 in interval
+    // t+1) stored from obuf[t % 2] coalesced
+    {
+      q32x4 up4[4];
+      load4(u_pos, nchunks - 1, up4);
 #pragma unroll
-    for (int t = NCH - 1; t >= -1; --t) {
-      if (t <= nchunks - 1) {
-        if (t >= 0) {
+      for (int t = NCH - 1; t >= -2; --t) {
+        if (t <= nchunks - 1) {
+          if (t >= 0) {
 #pragma unroll
-          for (int l = 0; l < L; ++l)
-            if (t * L + l < NLEV - 1) ring[t % 2][l * kWave + lane] = cpr[t * L + l];
+            for (int l = 0; l < L; ++l)
+              if (t * L + l < NLEV - 1) ring[t % 2][l * kWave + lane] = cpr[t * L + l];
+            stage(t1[t % 2][2], up4);
+            if (t >= 1) load4(u_pos, t - 1, up4);
+          }
+          const int qs = t + 2;
+          if (qs <= NCH - 1 && qs < nchunks) {
+            const q32x4* ob = t0[0][qs % 2];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+              const int64_t col = c0 + 16 * m + lcol;
+              if (col < ncols)
+                *reinterpret_cast<q32x4*>(utens_stage + col * K + qs * L + lq * V) =
+                    ob[(16 * m + lcol) * 4 + (lq ^ sw_w)];
+            }
+          }
+          __syncthreads();
         }
-        __syncthreads();
       }
     }
     return;
@@ -1074,22 +1100,16 @@ __global__ __launch_bounds__(2 * kWave) void vadv_stage(S* __restrict__ utens_st
     }
   }
 
-  // ---- backward (vadv_numpy.py:70-78): u_pos staged through T1, c' from
-  // the ring, the output through the wcon buffers (free now) for
-  // coalesced stores
-  q32x4 up4[4];
-  load4(u_pos, nchunks - 1, up4);
-  q32x4* obuf = t0[0][0];
+  // ---- backward (vadv_numpy.py:70-78), interval t: chunk t+1 solved from
+  // d' (registers), c' (ring) and u_pos (staged by wave 0), its output
+  // written to obuf[(t+1) % 2] for wave 0 to store
   double x = 0;
 #pragma unroll
-  for (int t = NCH - 1; t >= -1; --t) {
+  for (int t = NCH - 1; t >= -2; --t) {
     if (t <= nchunks - 1) {
-      if (t >= 0) {
-        stage(t1[t % 2][2], up4);
-        if (t >= 1) load4(u_pos, t - 1, up4);
-      }
       const int q = t + 1;  // the chunk solved in this interval
-      if (q <= NCH - 1 && q < nchunks) {
+      if (q >= 0 && q <= NCH - 1 && q < nchunks) {
+        q32x4* ob = t0[0][q % 2];
         const bool top = q == nchunks - 1;
 #pragma unroll
         for (int j = 3; j >= 0; --j) {
@@ -1110,14 +1130,7 @@ __global__ __launch_bounds__(2 * kWave) void vadv_stage(S* __restrict__ utens_st
           }
           q32x4 v;
           __builtin_memcpy(&v, o, 16);
-          obuf[lane * 4 + (j ^ sw_r)] = v;
-        }
-#pragma unroll
-        for (int m = 0; m < 4; ++m) {
-          const int64_t col = c0 + 16 * m + lcol;
-          if (col < ncols)
-            *reinterpret_cast<q32x4*>(utens_stage + col * K + q * L + lq * V) =
-                obuf[(16 * m + lcol) * 4 + (lq ^ sw_w)];
+          ob[lane * 4 + (j ^ sw_r)] = v;
         }
       }
       __syncthreads();
