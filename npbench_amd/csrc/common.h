@@ -149,6 +149,10 @@ void set_hdiff_tblock(int tb);
 // {columns per lane, rows per thread, bytes of k per lane}, hdiff.hip).
 int hdiff_shape();
 void set_hdiff_shape(int shape);
+// Output rows per workgroup strip of the two-step fused hdiff
+// ("hdiff.fuse_rows", 4 .. 256).
+int hdiff_fuse_rows();
+void set_hdiff_fuse_rows(int rows);
 int hdiff_shape_count();
 // ADI launch path ("adi.path"; 0 = by size, 1 = every time step in one
 // single-workgroup launch, 2 = one launch per half step).

@@ -464,6 +464,228 @@ __global__ __launch_bounds__(256) void hdiff_tb(const S* __restrict__ src,
   }
 }
 
+// ---------------------------------------------------------------- two steps
+// hdiff_fuse2<S, VB, BAND, KV>: two iterated steps in one pass, without
+// staging whole blocks.  A workgroup owns BAND in-field columns q = y0 ..
+// y0+BAND-1 (y0 = band (BAND-4)) at KV consecutive k-vectors of VB bytes
+// (lane = (column, k-vector), k fastest, so a wave's loads are 16 columns'
+// contiguous 64-byte k-segments) and marches down a strip of rows.  Each
+// lane runs hdiff_march's one-column march twice over: the first step at
+// in-field row p1 from the input, the second at row p1 - 2 from first-step
+// values -- its own column's from its registers, the two columns either
+// side of it from neighbouring lanes through a four-row LDS ring (written
+// before and read after one barrier per row).  First-step values outside
+// the field's interior are the input's (the halo the iterated form holds
+// fixed), interior ones are rounded through the storage type as the
+// iterated form's stores round them, and every Laplacian and flux has the
+// reference's operands in the reference's order, so the output is bitwise
+// two launches of hdiff_march.  The band's outer two columns on each side
+// give first-step values only (BAND - 4 columns of output), and a strip of
+// S output rows computes S + 4 first-step rows.
+template <class S, int VB, int BAND, int KV>
+__global__ __launch_bounds__(BAND * KV) void hdiff_fuse2(const S* __restrict__ src,
+                                                         const S* __restrict__ coeff,
+                                                         S* __restrict__ out, S* __restrict__ next,
+                                                         int I, int J, int K, int bands,
+                                                         int kgroups, int strip) {
+  using T = Storage<S>;
+  using C = typename T::C;
+  constexpr int V = VB / static_cast<int>(sizeof(S));
+  using Raw = RawVec<S, V>;
+  __shared__ C ring[4][BAND * KV * V];  // first-step rows: [row % 4][column][k-vector][v]
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const int band = b % bands, rest = b / bands;
+  const int kg = rest % kgroups, sidx = rest / kgroups;
+  const int cl = threadIdx.x / KV, kv = threadIdx.x % KV;
+  const int y0 = band * (BAND - 4);
+  const int q = y0 + cl;  // this lane's in-field column
+  const int kraw = (kg * KV + kv) * V;
+  const bool kok = kraw + V <= K;
+  const int k0 = kok ? kraw : K - V;  // lanes past K compute a copy and store nothing
+  const int x0 = sidx * strip, x1 = min(x0 + strip, I);  // second-step output rows
+  const int64_t si = static_cast<int64_t>(J + 4) * K;
+  auto colo = [&](int qq) { return static_cast<int64_t>(min(max(qq, 0), J + 3)) * K + k0; };
+  // in-field columns q-2 .. q+2, clamped (clamped columns feed only values
+  // that are replaced or not stored)
+  const int64_t cq[5] = {colo(q - 2), colo(q - 1), colo(q), colo(q + 1), colo(q + 2)};
+  auto ld = [&](int p, int c, Raw& r) {
+    loadraw<S, V>(src + static_cast<int64_t>(min(max(p, 0), I + 3)) * si + cq[c], r);
+  };
+  const S* cfbase = coeff + static_cast<int64_t>(min(max(q - 2, 0), J - 1)) * K + k0;
+  auto ldcf = [&](int x, Raw& r) {
+    loadraw<S, V>(cfbase + static_cast<int64_t>(min(max(x, 0), I - 1)) * J * K, r);
+  };
+  auto lap = [](C c, C ip, C im, C jp, C jm) {
+    C nb = ip + im;
+    nb = nb + jp;
+    nb = nb + jm;
+    return static_cast<C>(4.0) * c - nb;
+  };
+  const bool col_in = q >= 2 && q < J + 2;
+  // one row of the one-column march: own[r] = rows p, p+1, p+2 of the
+  // column, sl/sr[r] = rows p-1, p, p+1 of columns q-1 / q+1, fl/fr = row p
+  // of q-2 / q+2; L0, fx carried
+  auto march_row = [&](const C (&own)[3][V], const C (&sl)[3][V], const C (&sr)[3][V],
+                       const C (&fl)[V], const C (&fr)[V], const C (&cf)[V], C (&L0)[V],
+                       C (&fx)[V], C (&res)[V]) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const C c = own[0][v];
+      const C lcl = lap(sl[1][v], sl[2][v], sl[0][v], c, fl[v]);
+      const C lcr = lap(sr[1][v], sr[2][v], sr[0][v], fr[v], c);
+      const C fy0 = limit<C>(L0[v] - lcl, c - sl[1][v]);
+      const C fy1 = limit<C>(lcr - L0[v], sr[1][v] - c);
+      const C Lip = lap(own[1][v], own[2][v], c, sr[2][v], sl[2][v]);
+      const C flx_hi = limit<C>(Lip - L0[v], own[1][v] - c);
+      C dv = flx_hi - fx[v];
+      dv = dv + fy1;
+      dv = dv - fy0;
+      res[v] = c - cf[v] * dv;
+      L0[v] = Lip;
+      fx[v] = flx_hi;
+    }
+  };
+  // march start at row P: L0 = lap(P), fx = flux across (P-1 | P)
+  auto march_init = [&](const C (&om2)[V], const C (&om1)[V], const C (&o0)[V],
+                        const C (&o1)[V], const C (&slm)[V], const C (&sl0)[V],
+                        const C (&srm)[V], const C (&sr0)[V], C (&L0)[V], C (&fx)[V]) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      L0[v] = lap(o0[v], o1[v], om1[v], sr0[v], sl0[v]);
+      const C Lim = lap(om1[v], o0[v], om2[v], srm[v], slm[v]);
+      fx[v] = limit<C>(L0[v] - Lim, o0[v] - om1[v]);
+    }
+  };
+  auto ring_at = [&](int row, int dc) {  // row's slot, column cl + dc (clamped into the band)
+    const int c = min(max(cl + dc, 0), BAND - 1);
+    return &ring[row & 3][(c * KV + kv) * V];
+  };
+  auto ring_ld = [&](int row, int dc, C (&v)[V]) {
+    const C* p = ring_at(row, dc);
+#pragma unroll
+    for (int e = 0; e < V; ++e) v[e] = p[e];
+  };
+
+  // ---- first step: state at row x0
+  C o1s[3][V], sl1[3][V], sr1[3][V], fl1[V], fr1[V], L01[V], fx1[V];
+  {
+    Raw r[4], rl[2], rr[2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) ld(x0 - 2 + i, 2, r[i]);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      ld(x0 - 1 + i, 1, rl[i]);
+      ld(x0 - 1 + i, 3, rr[i]);
+    }
+    C om2[V], om1[V];
+    unpackv<S, V>(r[0], om2);
+    unpackv<S, V>(r[1], om1);
+    unpackv<S, V>(r[2], o1s[0]);
+    unpackv<S, V>(r[3], o1s[1]);
+    unpackv<S, V>(rl[0], sl1[0]);
+    unpackv<S, V>(rl[1], sl1[1]);
+    unpackv<S, V>(rr[0], sr1[0]);
+    unpackv<S, V>(rr[1], sr1[1]);
+    march_init(om2, om1, o1s[0], o1s[1], sl1[0], sl1[1], sr1[0], sr1[1], L01, fx1);
+  }
+  Raw nown, nsl, nsr, nfl, nfr, ncf;
+  ld(x0 + 2, 2, nown);
+  ld(x0 + 1, 1, nsl);
+  ld(x0 + 1, 3, nsr);
+  ld(x0, 0, nfl);
+  ld(x0, 4, nfr);
+  ldcf(x0 - 2, ncf);
+  C cfh[3][V];  // coefficients of the last three first-step rows (row p1 in [0], p1-1, p1-2)
+  // one first-step row p1: its value at column q, into the ring
+  auto step1 = [&](int p1, C (&s1)[V]) {
+    C cf[V];
+    unpackv<S, V>(nown, o1s[2]);
+    unpackv<S, V>(nsl, sl1[2]);
+    unpackv<S, V>(nsr, sr1[2]);
+    unpackv<S, V>(nfl, fl1);
+    unpackv<S, V>(nfr, fr1);
+    unpackv<S, V>(ncf, cf);
+    ld(p1 + 3, 2, nown);  // next row's loads (rows clamped into the field)
+    ld(p1 + 2, 1, nsl);
+    ld(p1 + 2, 3, nsr);
+    ld(p1 + 1, 0, nfl);
+    ld(p1 + 1, 4, nfr);
+    ldcf(p1 - 1, ncf);
+    C res[V];
+    march_row(o1s, sl1, sr1, fl1, fr1, cf, L01, fx1, res);
+    const bool in = col_in && p1 >= 2 && p1 < I + 2;
+#pragma unroll
+    for (int v = 0; v < V; ++v) s1[v] = in ? T::load(T::store(res[v])) : o1s[0][v];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      o1s[0][v] = o1s[1][v];
+      o1s[1][v] = o1s[2][v];
+      sl1[0][v] = sl1[1][v];
+      sl1[1][v] = sl1[2][v];
+      sr1[0][v] = sr1[1][v];
+      sr1[1][v] = sr1[2][v];
+      cfh[2][v] = cfh[1][v];
+      cfh[1][v] = cfh[0][v];
+      cfh[0][v] = cf[v];
+    }
+    C* w = ring_at(p1, 0);
+#pragma unroll
+    for (int e = 0; e < V; ++e) w[e] = s1[e];
+  };
+
+  // ---- warm-up: first-step rows x0 .. x0+3, then the second step's state
+  // at row P = x0+2
+  C h[4][V];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    step1(x0 + i, h[i]);
+    __syncthreads();
+  }
+  C o2s[3][V], sl2[3][V], sr2[3][V], fl2[V], fr2[V], L02[V], fx2[V];
+  ring_ld(x0 + 1, -1, sl2[0]);
+  ring_ld(x0 + 2, -1, sl2[1]);
+  ring_ld(x0 + 1, 1, sr2[0]);
+  ring_ld(x0 + 2, 1, sr2[1]);
+  march_init(h[0], h[1], h[2], h[3], sl2[0], sl2[1], sr2[0], sr2[1], L02, fx2);
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    o2s[0][v] = h[2][v];
+    o2s[1][v] = h[3][v];
+  }
+  const bool store_col = kok && cl >= 2 && cl < BAND - 2 && q < J + 2;
+  // ---- main: first-step row p1, second-step row p2 = p1 - 2
+  for (int p1 = x0 + 4; p1 < x1 + 4; ++p1) {
+    C s1[V];
+    step1(p1, s1);
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < V; ++v) o2s[2][v] = s1[v];
+    ring_ld(p1 - 1, -1, sl2[2]);
+    ring_ld(p1 - 1, 1, sr2[2]);
+    ring_ld(p1 - 2, -2, fl2);
+    ring_ld(p1 - 2, 2, fr2);
+    C res[V];
+    // coefficient of output row p2 - 2 = p1 - 4: the first step's at row p1 - 2
+    march_row(o2s, sl2, sr2, fl2, fr2, cfh[2], L02, fx2, res);
+    const int p2 = p1 - 2;
+    if (store_col) {
+      if (next)
+        storev<S, V>(next + static_cast<int64_t>(p2) * si + cq[2], res);
+      else
+        storev<S, V>(out + (static_cast<int64_t>(p2 - 2) * J + (q - 2)) * K + k0, res);
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      o2s[0][v] = o2s[1][v];
+      o2s[1][v] = o2s[2][v];
+      sl2[0][v] = sl2[1][v];
+      sl2[1][v] = sl2[2][v];
+      sr2[0][v] = sr2[1][v];
+      sr2[1][v] = sr2[2][v];
+    }
+  }
+}
+
 // Fused steps per round (tuning knob "hdiff.tblock"; 1 = one launch per step,
 // the default: at 256x256x160 on the MI355X the fused kernel is slower, its
 // lanes each reading 16 bytes of a different 640-byte k-column).
@@ -537,6 +759,23 @@ int run_hdiff(const S* in, S* out, const S* coeff, int I, int J, int K, int64_t 
                        K);
     NPBX_LAUNCHED("hdiff_copy_halo");
   }
+  // Two steps in one pass (hdiff_fuse2): 64 columns (60 of output) x 4
+  // k-vectors per workgroup, strips of hdiff.fuse_rows output rows.
+  constexpr int kFuseBand = 64, kFuseKV = 4;
+  constexpr int kFuseVB = sizeof(S) == 2 ? 8 : 16;
+  constexpr int kFuseV = kFuseVB / static_cast<int>(sizeof(S));
+  auto fuse2 = [&](const S* from, S* to) -> int {
+    const int bands = static_cast<int>(cdiv(J, kFuseBand - 4));
+    const int kgroups = static_cast<int>(cdiv(K / kFuseV, kFuseKV));
+    const int strip = hdiff_fuse_rows();
+    const int64_t nb = int64_t(bands) * kgroups * cdiv(I, strip);
+    NPBX_REQUIRE(nb < (int64_t(1) << 31), "hdiff: grid too large");
+    hipLaunchKernelGGL((hdiff_fuse2<S, kFuseVB, kFuseBand, kFuseKV>),
+                       dim3(static_cast<unsigned>(nb)), dim3(kFuseBand * kFuseKV), 0, stream, from,
+                       coeff, out, to, I, J, K, bands, kgroups, strip);
+    NPBX_LAUNCHED("hdiff_fuse2");
+    return NPBX_OK;
+  };
   const S* src = in;
   if (tb >= 2 && aligned && K % Vec16<S>::N == 0) {  // whole 16-byte vectors per column
     constexpr int V = Vec16<S>::N;
@@ -554,10 +793,12 @@ int run_hdiff(const S* in, S* out, const S* coeff, int I, int J, int K, int64_t 
       };
       if (tb >= 4) {
         if constexpr (hdiff_tb_max<S>() >= 4) go(hdiff_tb<S, 4>);
+        NPBX_LAUNCHED("hdiff_tb");
+      } else if (steps == 2) {
+        NPBX_TRY(fuse2(src, next));
       } else {
-        go(hdiff_tb<S, 2>);
+        NPBX_TRY(launch(src, next));
       }
-      NPBX_LAUNCHED("hdiff_tb");
       src = next;
       done += steps;
       ++round;
@@ -577,6 +818,9 @@ int run_hdiff(const S* in, S* out, const S* coeff, int I, int J, int K, int64_t 
 
 int hdiff_tblock() { return g_hdiff_tblock.load(); }
 void set_hdiff_tblock(int tb) { g_hdiff_tblock.store(tb); }
+std::atomic<int> g_hdiff_fuse_rows{16};
+int hdiff_fuse_rows() { return g_hdiff_fuse_rows.load(); }
+void set_hdiff_fuse_rows(int rows) { g_hdiff_fuse_rows.store(rows); }
 int hdiff_shape() { return g_hdiff_shape.load(); }
 void set_hdiff_shape(int shape) { g_hdiff_shape.store(shape); }
 int hdiff_shape_count() { return kMarchShapeCount; }

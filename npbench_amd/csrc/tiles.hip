@@ -1446,6 +1446,11 @@ extern "C" int npbx_set_tuning(const char* key, int64_t value) {
     set_jacobi_deep_depth(static_cast<int>(value));
     return NPBX_OK;
   }
+  if (std::strcmp(key, "hdiff.fuse_rows") == 0) {
+    NPBX_REQUIRE(value >= 4 && value <= 256, "set_tuning: hdiff.fuse_rows must be in [4, 256]");
+    set_hdiff_fuse_rows(static_cast<int>(value));
+    return NPBX_OK;
+  }
   if (std::strcmp(key, "hdiff.shape") == 0) {
     NPBX_REQUIRE(value >= 0 && value < hdiff_shape_count(),
                  "set_tuning: hdiff.shape must be in [0, %d]", hdiff_shape_count() - 1);
@@ -1471,7 +1476,7 @@ extern "C" int npbx_set_tuning(const char* key, int64_t value) {
   }
   return fail(NPBX_EINVAL,
               "set_tuning: unknown key '%s' (heat.rows, heat.seg, sweep.shape, hdiff.tblock, "
-              "hdiff.shape, adi.path, vadv.path, seidel.blocks, jacobi.deep, jacobi.deep_waves)",
+              "hdiff.shape, hdiff.fuse_rows, adi.path, vadv.path, seidel.blocks, jacobi.deep, jacobi.deep_waves)",
               key);
 }
 
@@ -1499,6 +1504,10 @@ extern "C" int npbx_get_tuning(const char* key, int64_t* value) {
   }
   if (std::strcmp(key, "jacobi.deep") == 0) {
     *value = jacobi_deep_depth();
+    return NPBX_OK;
+  }
+  if (std::strcmp(key, "hdiff.fuse_rows") == 0) {
+    *value = hdiff_fuse_rows();
     return NPBX_OK;
   }
   if (std::strcmp(key, "hdiff.shape") == 0) {

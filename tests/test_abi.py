@@ -85,6 +85,8 @@ def test_version_string():
     (lambda: nat.call("npbx_seidel2d", 0, None, 10, 3, None), "null"),
     (lambda: nat.call("npbx_set_tuning", b"seidel.blocks", 0), "seidel.blocks"),
     (lambda: nat.call("npbx_set_tuning", b"jacobi.deep", 9), "jacobi.deep"),
+    (lambda: nat.call("npbx_set_tuning", b"hdiff.fuse_rows", 3), "hdiff.fuse_rows"),
+    (lambda: nat.call("npbx_set_tuning", b"hdiff.shape", 14), "hdiff.shape"),
     (lambda: nat.call("npbx_set_tuning", b"jacobi.deep_waves", 2), "jacobi.deep_waves"),
     (lambda: nat.call("npbx_adi_noalloc", 0, None, 10, 5, None), "null"),
     (lambda: nat.call("npbx_adi_noalloc", 1, None, 10, 5, None), "dtype"),

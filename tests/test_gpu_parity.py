@@ -197,6 +197,37 @@ def test_hdiff_reduced_precision(st, precision, kernel_tol, f64_tol):
         assert err_f64 <= f64_tol
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16"])
+@pytest.mark.parametrize("rows", [4, 7, 16, 40])
+@pytest.mark.parametrize("I,J,K", [(9, 61, 16), (33, 130, 24), (70, 5, 8)])
+def test_hdiff_two_step_strips(st, hdiff_tblock, precision, rows, I, J, K):
+    """The two-step fused pass (hdiff.tblock 2) with strips of 4 to 40
+    output rows (hdiff.fuse_rows), bands of 60 output columns cut by the
+    grid edge and K-vectors past K: bitwise the one-launch-per-step
+    iteration at 4 and 5 steps (a last round of one step)."""
+    from npbench_amd import _native
+    dt = {"fp64": torch.float64, "fp32": torch.float32, "bf16": torch.bfloat16}[precision]
+    inf, out, coeff = _hdiff_inputs(I, J, K, 5 * I + J + K + rows)
+    d_in, d_c = dev(inf, dt), dev(coeff, dt)
+    old = _native.get_tuning("hdiff.fuse_rows")
+    try:
+        for nsteps in (4, 5):
+            print(f"hdiff two-step {precision} rows={rows} {I}x{J}x{K} nsteps={nsteps}", flush=True)
+            hdiff_tblock(1)
+            it = dev(out, dt)
+            st.hdiff(d_in, it, d_c, nsteps=nsteps)
+            hdiff_tblock(2)
+            _native.set_tuning("hdiff.fuse_rows", rows)
+            fu = dev(out, dt)
+            st.hdiff(d_in, fu, d_c, nsteps=nsteps)
+            torch.cuda.synchronize()
+            same = torch.equal(fu.view(torch.int16), it.view(torch.int16)) \
+                if dt == torch.bfloat16 else torch.equal(fu, it)
+            assert same, f"nsteps {nsteps}"
+    finally:
+        _native.set_tuning("hdiff.fuse_rows", old)
+
+
 @pytest.fixture
 def hdiff_shape():
     """Sets the library's hdiff.shape knob for one test and restores it."""
@@ -449,7 +480,8 @@ def hdiff_tblock():
 @pytest.mark.parametrize("I,J,K", [(1, 1, 8), (20, 24, 40), (70, 33, 16), (64, 96, 24)])
 @pytest.mark.parametrize("nsteps", [2, 3, 5, 8])
 def test_hdiff_fused_matches_iterated(st, hdiff_tblock, precision, tb, I, J, K, nsteps):
-    """Fused steps (hdiff.tblock 2 / 4, 32 x 32 blocks staged in LDS) give
+    """Fused steps (hdiff.tblock 2: the two-step register march; 4: 32 x 32
+    blocks staged in LDS) give
     bitwise the one-launch-per-step iteration, for blocks cut by the grid
     edge, rounds shorter than tblock and every storage type; FP64 is also
     bitwise the oracle's iteration."""
