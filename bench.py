@@ -223,6 +223,18 @@ def serving(torch, args, heat_rows=SERVE_HEAT_ROWS, cpu_rates=None, tblock=SERVE
                 "side U[32,256], steps U[10,200], FP32, seed 42", "tblock": tblock,
                 "heat_rows": heat_rows, "heat_seg": SERVE_SEG, "peak_active": srv.peak_active,
                 "ticks": srv.ticks})
+    # The rate above is the arrivals' (the offered load): the server keeps
+    # up.  Its capacity: the same requests all offered at once, admitted as
+    # memory and slots allow.
+    lups = sum(r.lups for r in reqs)
+    span = max(r.arrival for r in reqs) if reqs else 0.0
+    out["offered_glups"] = round(lups / span / 1e9, 2) if span > 0 else None
+    t0 = time.perf_counter()
+    srv.run(reqs, realtime=False)
+    cap_wall = time.perf_counter() - t0
+    out["capacity"] = {"glups": round(lups / cap_wall / 1e9, 2), "wall_s": round(cap_wall, 4),
+                       "note": "the same requests offered at t = 0 (no arrival waits): "
+                               "the server's rate on this mix"}
     srv.close()
     if cpu_rates:
         out["cpu_reference"] = cpu_serving_reference(reqs, cpu_rates)
