@@ -159,7 +159,8 @@ int hdiff_shape_count();
 int adi_path();
 void set_adi_path(int path);
 // vadv kernel choice ("vadv.path"; 0 = by shape, 1 = the workspace kernel,
-// 2 = the on-chip kernel where the shape allows it).
+// 2 = c' and d' in LDS, 3 = c' in registers, 4 = coalesced staged loads with
+// c' and d' in registers; 2 .. 4 where the shape allows them).
 int vadv_path();
 void set_vadv_path(int path);
 // Slot-blocks per seidel_2d launch ("seidel.blocks"; a slot-block is one

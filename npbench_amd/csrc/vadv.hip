@@ -817,25 +817,34 @@ __global__ __launch_bounds__(2 * kWave) void vadv_reg(S* __restrict__ utens_stag
 // column through LDS (a column's four 16-byte quads at XOR-swizzled slots,
 // so both the coalesced writes and the per-column reads are free of bank
 // conflicts).  Wave 0 keeps c'_k in registers, wave 1 keeps d'_k in
-// registers, so LDS holds only the staged chunks and two small rings (inv_k
-// from wave 0 to wave 1 on the way down, c'_k on the way back up): 73 KiB
-// (FP32) or 65 KiB (FP64) per 64 columns, two workgroups per CU.  K is a
-// whole number of chunks (L = 64 bytes of levels), so a chunk's levels run
-// with no branch between them and its ring entries are read together.
+// registers, so LDS holds only the staged chunks and three small rings
+// (inv_k and cs_k from wave 0 to wave 1 on the way down, c'_k on the way
+// back up): 80 KiB (FP32) or 64 KiB (FP64) per 64 columns, two workgroups
+// per CU.  Wave 1 needs no wcon: the reference's cs_k = gcv_k BET_M equals
+// wave 0's ccol coefficient gcv_k BET_P (BET_M = BET_P), and as_k = acol_k
+// = gav_k BET_M = -cs_{k-1} exactly (gav_k = -0.25 s_k and gcv_{k-1} =
+// 0.25 s_k differ only in sign, and rounding is symmetric).  K is a whole
+// number of chunks (L = 64 bytes of levels), so a chunk's levels run with no
+// branch between them and its ring entries are read together; NCHK > 0
+// fixes the chunk count at compile time (K = 160), which removes every
+// branch between the intervals and lets the compiler count the loads in
+// flight exactly (with a runtime count it waits for all of them).
 //   forward, interval n (n = 0 .. nchunks, a barrier after each):
-//     wave 0 stages wcon's chunk n+1 (T0, three buffers), loads chunk n+3,
-//       and runs the pivot chain of chunk n from T0, inv_k into ring[n % 2];
+//     wave 0 stages wcon's chunk n+1 (T0, two buffers), loads chunk n+3,
+//       and runs the pivot chain of chunk n from T0, inv_k and cs_k into
+//       the rings' buffer n % 2;
 //     wave 1 stages its four fields' chunk n (T1, two buffers), loads chunk
-//       n+2, and forms d'_k of chunk n-1 from T1, T0 and ring[(n-1) % 2]
-//   (loads two intervals ahead, in two register sets);
-//   backward, interval t (t = nchunks-1 .. -1):
-//     wave 0 writes c'_k of chunk t into ring[t % 2];
-//     wave 1 stages u_pos's chunk t, loads chunk t-1, and solves chunk t+1
-//       (x_k = d'_k - c'_k x_{k+1}, out = dtr (x - u_pos)), the output
-//       turned back to the coalesced layout through LDS for the stores.
+//       n+1, and forms d'_k of chunk n-1 from T1 and the rings' buffer
+//       (n-1) % 2;
+//   backward, interval t (t = nchunks-1 .. -2):
+//     wave 0 writes c'_k of chunk t into ring[t % 2], stages u_pos's chunk
+//       t (loaded an interval ahead) and stores the output of chunk t+2
+//       from LDS, coalesced;
+//     wave 1 solves chunk t+1 (x_k = d'_k - c'_k x_{k+1}, out = dtr (x -
+//       u_pos)) into an output buffer in LDS, lane = column.
 // Chunk loops are unrolled (register arrays need constant indices); the
 // arithmetic is vadv_reg's, operation for operation.
-template <class S>
+template <class S, int NCHK>
 __global__ __launch_bounds__(2 * kWave) void vadv_stage(S* __restrict__ utens_stage,
                                                         const S* __restrict__ u_stage,
                                                         const S* __restrict__ wcon,
@@ -846,12 +855,18 @@ __global__ __launch_bounds__(2 * kWave) void vadv_stage(S* __restrict__ utens_st
   constexpr int L = 64 / static_cast<int>(sizeof(S));  // levels per chunk
   constexpr int V = 16 / static_cast<int>(sizeof(S));  // levels per 16-byte quad
   constexpr int NCH = NLEV / L;
-  __shared__ q32x4 t0[3][2][4 * kWave];  // wcon rows i, i+1: [buffer][stream][slot]
+  __shared__ q32x4 t0[2][2][4 * kWave];  // wcon rows i, i+1: [buffer][stream][slot]
   __shared__ q32x4 t1[2][4][4 * kWave];  // utens_stage, u_stage, u_pos, utens
-  __shared__ double ring[2][(L + 1) * kWave];
+  __shared__ double ring[2][L * kWave];     // inv_k down, c'_k back up
+  __shared__ double ring_cs[2][L * kWave];  // cs_k down
+  static_assert(sizeof(t0) + sizeof(t1) + sizeof(ring) + sizeof(ring_cs) <= 80 * 1024,
+                "two workgroups per CU");
   const int lane = threadIdx.x % kWave;
   const int64_t c0 = static_cast<int64_t>(xcd_tile(blockIdx.x, gridDim.x)) * kWave;
-  const int nchunks = K / L;
+  const int nchunks = NCHK > 0 ? NCHK : K / L;
+  // level K-1's inv_k, wave 0 to wave 1 in the last forward interval: the
+  // t0 buffer no chunk is staged into then
+  double* const fin = reinterpret_cast<double*>(t0[nchunks % 2][0]);
   // coalesced role: load m of a stream's chunk gives lane g quad g % 4 of
   // column 16 m + g / 4 (columns past the last read the last column)
   const int lq = lane & 3, lcol = lane >> 2;
@@ -899,8 +914,8 @@ __global__ __launch_bounds__(2 * kWave) void vadv_stage(S* __restrict__ utens_st
     for (int n = 0; n <= NCH; ++n) {
       if (n <= nchunks) {
         if (n + 1 < nchunks) {  // stage chunk n+1, load chunk n+3 into its set
-          stage(t0[(n + 1) % 3][0], ra[(n + 1) % 2]);
-          stage(t0[(n + 1) % 3][1], rb[(n + 1) % 2]);
+          stage(t0[(n + 1) % 2][0], ra[(n + 1) % 2]);
+          stage(t0[(n + 1) % 2][1], rb[(n + 1) % 2]);
           if (n + 3 < nchunks) {
             load4(w0, n + 3, ra[(n + 1) % 2]);
             load4(w1, n + 3, rb[(n + 1) % 2]);
@@ -911,6 +926,7 @@ __global__ __launch_bounds__(2 * kWave) void vadv_stage(S* __restrict__ utens_st
           // pivot left its range, the chunk again from the same state with
           // the division operator (bitwise the short form where it holds).
           double* rg = ring[n % 2];
+          double* rc = ring_cs[n % 2];
           auto levels = [&](auto full) {
             double lo = INFINITY, hi = 0.0;
             auto recip = [&](double den) {
@@ -924,8 +940,8 @@ __global__ __launch_bounds__(2 * kWave) void vadv_stage(S* __restrict__ utens_st
               double sw[V];
               {
                 double a[V];
-                quad(t0[n % 3][0], j, a);
-                quad(t0[n % 3][1], j, sw);
+                quad(t0[n % 2][0], j, a);
+                quad(t0[n % 2][1], j, sw);
 #pragma unroll
                 for (int e = 0; e < V; ++e) sw[e] = sw[e] + a[e];
               }
@@ -943,6 +959,7 @@ __global__ __launch_bounds__(2 * kWave) void vadv_stage(S* __restrict__ utens_st
                   c_prev = ck * inv;
                   cpr[0] = c_prev;
                   rg[l * kWave + lane] = inv;
+                  rc[l * kWave + lane] = ck;
                   s_cur = sw[e];
                 } else {  // level k-1 (vadv_numpy.py:32-53)
                   const double gav = -0.25 * s_cur;
@@ -954,6 +971,7 @@ __global__ __launch_bounds__(2 * kWave) void vadv_stage(S* __restrict__ utens_st
                   c_prev = ck * inv;
                   cpr[k - 1] = c_prev;
                   rg[l * kWave + lane] = inv;
+                  rc[l * kWave + lane] = ck;
                   s_cur = sw[e];
                 }
               }
@@ -962,7 +980,7 @@ __global__ __launch_bounds__(2 * kWave) void vadv_stage(S* __restrict__ utens_st
               const double gav = -0.25 * s_cur;
               const double acol = gav * kBetP;
               const double bcol = dtr - acol;
-              rg[L * kWave + lane] = recip(bcol - c_prev * acol);
+              fin[lane] = recip(bcol - c_prev * acol);
             }
             return reciprocal_range_ok(lo, hi);
           };
@@ -1013,45 +1031,42 @@ __global__ __launch_bounds__(2 * kWave) void vadv_stage(S* __restrict__ utens_st
 
   // ======== wave 1: the fields, d' in registers, the back substitution
   double dpr[NLEV];
-  q32x4 st[2][4][4];  // chunk c's fields in set c % 2
+  q32x4 st[4][4];  // the next chunk's fields
   const S* fields[4] = {utens_stage, u_stage, u_pos, utens};
 #pragma unroll
-  for (int f = 0; f < 4; ++f) load4(fields[f], 0, st[0][f]);
-#pragma unroll
-  for (int f = 0; f < 4; ++f) load4(fields[f], 1, st[1][f]);
-  double u_prev = 0, u_cur = 0, s_cur = 0, base_cur = 0, d_prev = 0;
+  for (int f = 0; f < 4; ++f) load4(fields[f], 0, st[f]);
+  double u_prev = 0, u_cur = 0, base_cur = 0, d_prev = 0, cs_prev = 0;
 #pragma unroll
   for (int n = 0; n <= NCH; ++n) {
     if (n <= nchunks) {
-      if (n < nchunks) {  // stage chunk n, load chunk n+2 into its set
+      if (n < nchunks) {  // stage chunk n, load chunk n+1
 #pragma unroll
-        for (int f = 0; f < 4; ++f) stage(t1[n % 2][f], st[n % 2][f]);
-        if (n + 2 < nchunks) {
+        for (int f = 0; f < 4; ++f) stage(t1[n % 2][f], st[f]);
+        if (n + 1 < nchunks) {
 #pragma unroll
-          for (int f = 0; f < 4; ++f) load4(fields[f], n + 2, st[n % 2][f]);
+          for (int f = 0; f < 4; ++f) load4(fields[f], n + 1, st[f]);
         }
       }
       if (n >= 1) {  // d' of chunk p = n-1
         const int p = n - 1;
         const double* rg = ring[p % 2];
+        const double* rc = ring_cs[p % 2];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {  // quad j
-          double u[V], sw[V], base[V], inv[V];
+          double u[V], base[V], inv[V], cs[V];
 #pragma unroll
-          for (int e = 0; e < V; ++e) inv[e] = rg[(j * V + e) * kWave + lane];
+          for (int e = 0; e < V; ++e) {
+            inv[e] = rg[(j * V + e) * kWave + lane];
+            cs[e] = rc[(j * V + e) * kWave + lane];
+          }
           {
-            double fs[V], fp[V], ft[V], a[V];
+            double fs[V], fp[V], ft[V];
             quad(t1[p % 2][0], j, fs);
             quad(t1[p % 2][1], j, u);
             quad(t1[p % 2][2], j, fp);
             quad(t1[p % 2][3], j, ft);
-            quad(t0[p % 3][0], j, a);
-            quad(t0[p % 3][1], j, sw);
 #pragma unroll
-            for (int e = 0; e < V; ++e) {
-              sw[e] = sw[e] + a[e];
-              base[e] = (dtr * fp[e] + ft[e]) + fs[e];
-            }
+            for (int e = 0; e < V; ++e) base[e] = (dtr * fp[e] + ft[e]) + fs[e];
           }
 #pragma unroll
           for (int e = 0; e < V; ++e) {
@@ -1059,41 +1074,32 @@ __global__ __launch_bounds__(2 * kWave) void vadv_stage(S* __restrict__ utens_st
             const int k = p * L + l;
             if (k == 0) {
               u_cur = u[e];
-              s_cur = sw[e];
               base_cur = base[e];
               continue;
             }
             if (k == 1) {  // level 0 (vadv_numpy.py:15-30)
-              const double gcv = 0.25 * sw[e];
-              const double cs = gcv * kBetM;
-              const double corr = -cs * (u[e] - u_cur);
+              const double corr = -cs[e] * (u[e] - u_cur);
               const double d = base_cur + corr;
               d_prev = d * inv[e];
               dpr[0] = d_prev;
             } else {  // level k-1
-              const double gav = -0.25 * s_cur;
-              const double gcv = 0.25 * sw[e];
-              const double as_ = gav * kBetM;
-              const double cs = gcv * kBetM;
-              const double acol = gav * kBetP;
-              const double corr = -as_ * (u_prev - u_cur) - cs * (u[e] - u_cur);
+              const double as_ = -cs_prev, acol = as_;
+              const double corr = -as_ * (u_prev - u_cur) - cs[e] * (u[e] - u_cur);
               const double d = base_cur + corr;
               d_prev = (d - d_prev * acol) * inv[e];
               dpr[k - 1] = d_prev;
             }
+            cs_prev = cs[e];
             u_prev = u_cur;
             u_cur = u[e];
-            s_cur = sw[e];
             base_cur = base[e];
           }
         }
         if (p == nchunks - 1) {  // level K-1 (vadv_numpy.py:55-68)
-          const double gav = -0.25 * s_cur;
-          const double as_ = gav * kBetM;
-          const double acol = gav * kBetP;
+          const double as_ = -cs_prev, acol = as_;
           const double corr = -as_ * (u_prev - u_cur);
           const double d = base_cur + corr;
-          dpr[p * L + L - 1] = (d - d_prev * acol) * rg[L * kWave + lane];
+          dpr[p * L + L - 1] = (d - d_prev * acol) * fin[lane];
         }
       }
       __syncthreads();
@@ -1167,8 +1173,15 @@ int run_vadv(S* utens_stage, const S* u_stage, const S* wcon, const S* u_pos, co
   }
   // vadv_stage: 80 < K <= 160, whole 64-byte chunks of levels per column
   if (vec && K > 80 && K <= kLdsMaxK && K % (64 / sizeof(S)) == 0 && (path == 4 || path == 0)) {
-    hipLaunchKernelGGL((vadv_stage<S>), grid, dim3(2 * kWave), 0, stream, utens_stage, u_stage,
-                       wcon, u_pos, utens, dtr, static_cast<int>(J), static_cast<int>(K), ncols);
+    constexpr int kFull = kLdsMaxK / (64 / static_cast<int>(sizeof(S)));
+    if (K == kLdsMaxK)
+      hipLaunchKernelGGL((vadv_stage<S, kFull>), grid, dim3(2 * kWave), 0, stream, utens_stage,
+                         u_stage, wcon, u_pos, utens, dtr, static_cast<int>(J),
+                         static_cast<int>(K), ncols);
+    else
+      hipLaunchKernelGGL((vadv_stage<S, 0>), grid, dim3(2 * kWave), 0, stream, utens_stage,
+                         u_stage, wcon, u_pos, utens, dtr, static_cast<int>(J),
+                         static_cast<int>(K), ncols);
     NPBX_LAUNCHED("vadv_stage");
     return NPBX_OK;
   }
