@@ -830,11 +830,12 @@ __global__ __launch_bounds__(2 * kWave) void vadv_reg(S* __restrict__ utens_stag
 // branch between the intervals and lets the compiler count the loads in
 // flight exactly (with a runtime count it waits for all of them).
 //   forward, interval n (n = 0 .. nchunks, a barrier after each):
-//     wave 0 stages wcon's chunk n+1 (T0, two buffers), loads chunk n+3,
-//       and runs the pivot chain of chunk n from T0, inv_k and cs_k into
-//       the rings' buffer n % 2;
-//     wave 1 stages its four fields' chunk n (T1, two buffers), loads chunk
-//       n+1, and forms d'_k of chunk n-1 from T1 and the rings' buffer
+//     wave 0 runs the pivot chain of chunk n from T0, inv_k and cs_k into
+//       the rings' buffer n % 2 (the chain's latency bounds the forward, so
+//       wave 0 does nothing else);
+//     wave 1 stages the four fields' chunk n (T1, two buffers) and wcon's
+//       chunk n+1 (T0, two buffers), loads the fields' chunk n+1 and wcon's
+//       n+2, and forms d'_k of chunk n-1 from T1 and the rings' buffer
 //       (n-1) % 2;
 //   backward, interval t (t = nchunks-1 .. -2):
 //     wave 0 writes c'_k of chunk t into ring[t % 2], stages u_pos's chunk
@@ -894,33 +895,13 @@ __global__ __launch_bounds__(2 * kWave) void vadv_stage(S* __restrict__ utens_st
   };
 
   if (threadIdx.x < kWave) {
-    // ======== wave 0: wcon, the pivot chain, c' in registers
-    const S* w0 = wcon;
-    const S* w1 = wcon + static_cast<int64_t>(J) * K;
+    // ======== wave 0: the pivot chain, c' in registers
     double cpr[NLEV];
-    q32x4 ra[2][4], rb[2][4];  // chunk c's wcon rows in set c % 2
-    load4(w0, 0, ra[0]);
-    load4(w1, 0, rb[0]);
-    stage(t0[0][0], ra[0]);
-    stage(t0[0][1], rb[0]);
-    load4(w0, 1, ra[1]);  // nchunks >= 2 (K > 80)
-    load4(w1, 1, rb[1]);
-    if (nchunks > 2) {
-      load4(w0, 2, ra[0]);
-      load4(w1, 2, rb[0]);
-    }
+    __syncthreads();  // wcon's chunk 0 staged
     double s_cur = 0, c_prev = 0;
 #pragma unroll
     for (int n = 0; n <= NCH; ++n) {
       if (n <= nchunks) {
-        if (n + 1 < nchunks) {  // stage chunk n+1, load chunk n+3 into its set
-          stage(t0[(n + 1) % 2][0], ra[(n + 1) % 2]);
-          stage(t0[(n + 1) % 2][1], rb[(n + 1) % 2]);
-          if (n + 3 < nchunks) {
-            load4(w0, n + 3, ra[(n + 1) % 2]);
-            load4(w1, n + 3, rb[(n + 1) % 2]);
-          }
-        }
         if (n < nchunks) {
           // The chunk's levels with the short reciprocal; if any lane's
           // pivot left its range, the chunk again from the same state with
@@ -1032,19 +1013,37 @@ __global__ __launch_bounds__(2 * kWave) void vadv_stage(S* __restrict__ utens_st
   // ======== wave 1: the fields, d' in registers, the back substitution
   double dpr[NLEV];
   q32x4 st[4][4];  // the next chunk's fields
+  q32x4 wt[2][4];  // wcon rows i, i+1 of the chunk after
   const S* fields[4] = {utens_stage, u_stage, u_pos, utens};
+  const S* wrow[2] = {wcon, wcon + static_cast<int64_t>(J) * K};
+  {
+    q32x4 w0[2][4];
 #pragma unroll
-  for (int f = 0; f < 4; ++f) load4(fields[f], 0, st[f]);
+    for (int r = 0; r < 2; ++r) load4(wrow[r], 0, w0[r]);
+#pragma unroll
+    for (int r = 0; r < 2; ++r) load4(wrow[r], 1, wt[r]);  // nchunks >= 2 (K > 80)
+#pragma unroll
+    for (int f = 0; f < 4; ++f) load4(fields[f], 0, st[f]);
+#pragma unroll
+    for (int r = 0; r < 2; ++r) stage(t0[0][r], w0[r]);
+  }
+  __syncthreads();  // wcon's chunk 0 staged
   double u_prev = 0, u_cur = 0, base_cur = 0, d_prev = 0, cs_prev = 0;
 #pragma unroll
   for (int n = 0; n <= NCH; ++n) {
     if (n <= nchunks) {
-      if (n < nchunks) {  // stage chunk n, load chunk n+1
+      if (n < nchunks) {  // stage the fields' chunk n and wcon's n+1; load n+1, n+2
 #pragma unroll
         for (int f = 0; f < 4; ++f) stage(t1[n % 2][f], st[f]);
         if (n + 1 < nchunks) {
 #pragma unroll
+          for (int r = 0; r < 2; ++r) stage(t0[(n + 1) % 2][r], wt[r]);
+#pragma unroll
           for (int f = 0; f < 4; ++f) load4(fields[f], n + 1, st[f]);
+        }
+        if (n + 2 < nchunks) {
+#pragma unroll
+          for (int r = 0; r < 2; ++r) load4(wrow[r], n + 2, wt[r]);
         }
       }
       if (n >= 1) {  // d' of chunk p = n-1
