@@ -886,12 +886,14 @@ __global__ __launch_bounds__(2 * kWave) void vadv_stage(S* __restrict__ utens_st
     for (int m = 0; m < 4; ++m) buf[(16 * m + lcol) * 4 + (lq ^ sw_w)] = r[m];
   };
   // quad j (V levels) of column `lane` of a staged chunk, as doubles
-  auto quad = [&](const q32x4* buf, int j, double (&out)[V]) {
-    const q32x4 q = buf[lane * 4 + (j ^ sw_r)];
+  auto conv = [&](const q32x4 q, double (&out)[V]) {
     S v[V];
     __builtin_memcpy(v, &q, 16);
 #pragma unroll
     for (int e = 0; e < V; ++e) out[e] = static_cast<double>(v[e]);
+  };
+  auto quad = [&](const q32x4* buf, int j, double (&out)[V]) {
+    conv(buf[lane * 4 + (j ^ sw_r)], out);
   };
 
   if (threadIdx.x < kWave) {
@@ -1116,21 +1118,27 @@ __global__ __launch_bounds__(2 * kWave) void vadv_stage(S* __restrict__ utens_st
       if (q >= 0 && q <= NCH - 1 && q < nchunks) {
         q32x4* ob = t0[0][q % 2];
         const bool top = q == nchunks - 1;
+        // the chunk's u_pos quads and c' values, all read before the chain
+        // starts (one LDS round trip per interval instead of two per quad)
+        q32x4 uq[4];
+        double cpv[L];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) uq[j] = t1[q % 2][2][lane * 4 + (j ^ sw_r)];
+#pragma unroll
+        for (int l = 0; l < L; ++l) cpv[l] = ring[q % 2][l * kWave + lane];
 #pragma unroll
         for (int j = 3; j >= 0; --j) {
-          double up[V], cpv[V];
-          quad(t1[q % 2][2], j, up);
-#pragma unroll
-          for (int e = 0; e < V; ++e) cpv[e] = ring[q % 2][(j * V + e) * kWave + lane];
+          double up[V];
+          conv(uq[j], up);
           S o[V];
 #pragma unroll
           for (int e = V - 1; e >= 0; --e) {
             const int l = j * V + e;
             const int k = q * L + l;
             if (l == L - 1)
-              x = top ? dpr[k] : dpr[k] - cpv[e] * x;
+              x = top ? dpr[k] : dpr[k] - cpv[l] * x;
             else
-              x = dpr[k] - cpv[e] * x;
+              x = dpr[k] - cpv[l] * x;
             o[e] = static_cast<S>(dtr * (x - up[e]));
           }
           q32x4 v;
