@@ -361,6 +361,34 @@ def test_vadv_bruteforce_bitwise(shape):
     assert np.array_equal(out, scalar.vadv(us, u, w, up, ut, 0.15))
 
 
+def test_vadv_coefficient_identities():
+    """The identities that let the GPU kernel hand one coefficient per level
+    from its pivot wave to its right-hand-side wave: cs_k = gcv_k BET_M is
+    the pivot chain's c_k = gcv_k BET_P (BET_M = BET_P), and as_k = acol_k =
+    gav_k BET_M = -cs_{k-1} bitwise, because gav_k = -0.25 s_k and gcv_{k-1}
+    = 0.25 s_k differ only in sign and round-to-nearest is symmetric.  Checked
+    over wcon sums of every magnitude, subnormals, signed zeros and
+    infinities (vadv_numpy.py:32-40)."""
+    assert S.BET_M == S.BET_P
+    rng = np.random.default_rng(3)
+    mant = rng.random(200_000) + 1.0
+    expo = rng.integers(-1074, 1024, 200_000)
+    s = np.ldexp(mant, expo) * rng.choice([-1.0, 1.0], 200_000)
+    s = np.concatenate([s, [0.0, -0.0, np.inf, -np.inf, 5e-324, -5e-324, 2.2250738585072014e-308,
+                            1.7976931348623157e308, -1.7976931348623157e308]])
+    with np.errstate(over="ignore", under="ignore"):
+        gav = -0.25 * s
+        gcv_prev = 0.25 * s
+        as_ = gav * S.BET_M
+        acol = gav * S.BET_P
+        cs_prev = gcv_prev * S.BET_M
+    neg = -cs_prev
+    assert np.array_equal(as_.view(np.uint64), neg.view(np.uint64))
+    assert np.array_equal(acol.view(np.uint64), neg.view(np.uint64))
+    # the correction term's -as_ is then cs_{k-1} itself
+    assert np.array_equal((-as_).view(np.uint64), cs_prev.view(np.uint64))
+
+
 def test_vadv_rejects_k1():
     z = np.zeros((1, 1, 1))
     with pytest.raises(ValueError):
