@@ -856,18 +856,24 @@ __global__ __launch_bounds__(2 * kWave) void vadv_stage(S* __restrict__ utens_st
   constexpr int L = 64 / static_cast<int>(sizeof(S));  // levels per chunk
   constexpr int V = 16 / static_cast<int>(sizeof(S));  // levels per 16-byte quad
   constexpr int NCH = NLEV / L;
-  __shared__ q32x4 t0[2][2][4 * kWave];  // wcon rows i, i+1: [buffer][stream][slot]
+  // wcon rows i, i+1: [buffer][stream][slot]; a buffer also carries level
+  // K-1's inv_k as doubles in the last forward interval (see fin)
+  __shared__ union {
+    q32x4 q[2][2][4 * kWave];
+    double d[2][2 * 4 * kWave * 2];
+  } t0u;
+  auto& t0 = t0u.q;
   __shared__ q32x4 t1[2][4][4 * kWave];  // utens_stage, u_stage, u_pos, utens
   __shared__ double ring[2][L * kWave];     // inv_k down, c'_k back up
   __shared__ double ring_cs[2][L * kWave];  // cs_k down
-  static_assert(sizeof(t0) + sizeof(t1) + sizeof(ring) + sizeof(ring_cs) <= 80 * 1024,
+  static_assert(sizeof(t0u) + sizeof(t1) + sizeof(ring) + sizeof(ring_cs) <= 80 * 1024,
                 "two workgroups per CU");
   const int lane = threadIdx.x % kWave;
   const int64_t c0 = static_cast<int64_t>(xcd_tile(blockIdx.x, gridDim.x)) * kWave;
   const int nchunks = NCHK > 0 ? NCHK : K / L;
   // level K-1's inv_k, wave 0 to wave 1 in the last forward interval: the
   // t0 buffer no chunk is staged into then
-  double* const fin = reinterpret_cast<double*>(t0[nchunks % 2][0]);
+  double* const fin = t0u.d[nchunks % 2];
   // coalesced role: load m of a stream's chunk gives lane g quad g % 4 of
   // column 16 m + g / 4 (columns past the last read the last column)
   const int lq = lane & 3, lcol = lane >> 2;
