@@ -357,11 +357,15 @@ def test_vadv_extreme_pivots(st, vadv_path, path, dtr, wscale):
 
 
 @pytest.mark.parametrize("I,J,K", [(3, 37, 12), (7, 11, 80), (5, 13, 44), (2, 64, 160),
+                                   (17, 9, 160), (3, 37, 96), (5, 13, 144),
                                    (2, 3, 200), (4, 5, 36), (3, 5, 7)])
 def test_vadv_fp32_fields_shapes(st, I, J, K):
     """FP32 fields through every kernel shape (on-chip 40, 80, 160 levels;
-    the workspace kernel above 160 levels or with partial vectors): within
-    the final FP32 rounding of the oracle on the same FP32 inputs."""
+    the staged kernel at K = 160 with its chunk count fixed at compile time
+    and at K = 96, 144 with a runtime count, each with a partial last
+    workgroup; the workspace kernel above 160 levels or with partial
+    vectors): within the final FP32 rounding of the oracle on the same FP32
+    inputs."""
     print(f"vadv fp32 fields {I}x{J}x{K}", flush=True)
     f = _vadv_inputs(I, J, K, 7 * I + J + K)
     q = [quantize(a, torch.float32) for a in f]
