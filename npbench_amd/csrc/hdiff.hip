@@ -27,6 +27,7 @@
 // workspace holding one (I+4, J+4, K) field; in_field is never written.
 #include <algorithm>
 #include <atomic>
+#include <type_traits>
 
 #include "common.h"
 #include "vecio.h"
@@ -522,21 +523,24 @@ __global__ __launch_bounds__(BAND * KV) void hdiff_fuse2(const S* __restrict__ s
     return static_cast<C>(4.0) * c - nb;
   };
   const bool col_in = q >= 2 && q < J + 2;
-  // one row of the one-column march: own[r] = rows p, p+1, p+2 of the
-  // column, sl/sr[r] = rows p-1, p, p+1 of columns q-1 / q+1, fl/fr = row p
-  // of q-2 / q+2; L0, fx carried
-  auto march_row = [&](const C (&own)[3][V], const C (&sl)[3][V], const C (&sr)[3][V],
-                       const C (&fl)[V], const C (&fr)[V], const C (&cf)[V], C (&L0)[V],
-                       C (&fx)[V], C (&res)[V]) {
+  // one row of the one-column march: own[s0], own[s1], own[s2] = rows p,
+  // p+1, p+2 of the column, sl/sr[s0 .. s2] = rows p-1, p, p+1 of columns
+  // q-1 / q+1 (s0 = the rotation rot, s1 = rot+1, s2 = rot+2, mod 3: the
+  // three-row windows rotate through fixed slots instead of being copied),
+  // fl/fr = row p of q-2 / q+2; L0, fx carried
+  auto march_row = [&](auto rot, const C (&own)[3][V], const C (&sl)[3][V],
+                       const C (&sr)[3][V], const C (&fl)[V], const C (&fr)[V],
+                       const C (&cf)[V], C (&L0)[V], C (&fx)[V], C (&res)[V]) {
+    constexpr int s0 = decltype(rot)::value % 3, s1 = (s0 + 1) % 3, s2 = (s0 + 2) % 3;
 #pragma unroll
     for (int v = 0; v < V; ++v) {
-      const C c = own[0][v];
-      const C lcl = lap(sl[1][v], sl[2][v], sl[0][v], c, fl[v]);
-      const C lcr = lap(sr[1][v], sr[2][v], sr[0][v], fr[v], c);
-      const C fy0 = limit<C>(L0[v] - lcl, c - sl[1][v]);
-      const C fy1 = limit<C>(lcr - L0[v], sr[1][v] - c);
-      const C Lip = lap(own[1][v], own[2][v], c, sr[2][v], sl[2][v]);
-      const C flx_hi = limit<C>(Lip - L0[v], own[1][v] - c);
+      const C c = own[s0][v];
+      const C lcl = lap(sl[s1][v], sl[s2][v], sl[s0][v], c, fl[v]);
+      const C lcr = lap(sr[s1][v], sr[s2][v], sr[s0][v], fr[v], c);
+      const C fy0 = limit<C>(L0[v] - lcl, c - sl[s1][v]);
+      const C fy1 = limit<C>(lcr - L0[v], sr[s1][v] - c);
+      const C Lip = lap(own[s1][v], own[s2][v], c, sr[s2][v], sl[s2][v]);
+      const C flx_hi = limit<C>(Lip - L0[v], own[s1][v] - c);
       C dv = flx_hi - fx[v];
       dv = dv + fy1;
       dv = dv - fy0;
@@ -595,13 +599,17 @@ __global__ __launch_bounds__(BAND * KV) void hdiff_fuse2(const S* __restrict__ s
   ld(x0, 0, nfl);
   ld(x0, 4, nfr);
   ldcf(x0 - 2, ncf);
-  C cfh[3][V];  // coefficients of the last three first-step rows (row p1 in [0], p1-1, p1-2)
-  // one first-step row p1: its value at column q, into the ring
-  auto step1 = [&](int p1, C (&s1)[V]) {
-    C cf[V];
-    unpackv<S, V>(nown, o1s[2]);
-    unpackv<S, V>(nsl, sl1[2]);
-    unpackv<S, V>(nsr, sr1[2]);
+  // coefficients of the last three first-step rows: row p1's in slot
+  // rot % 3 of its step (rot as in march_row)
+  C cfh[3][V];
+  // one first-step row p1 at rotation rot: its value at column q, into the
+  // ring
+  auto step1 = [&](auto rot, int p1, C (&s1)[V]) {
+    constexpr int s0 = decltype(rot)::value % 3, s2 = (s0 + 2) % 3;
+    C (&cf)[V] = cfh[s0];
+    unpackv<S, V>(nown, o1s[s2]);
+    unpackv<S, V>(nsl, sl1[s2]);
+    unpackv<S, V>(nsr, sr1[s2]);
     unpackv<S, V>(nfl, fl1);
     unpackv<S, V>(nfr, fr1);
     unpackv<S, V>(ncf, cf);
@@ -612,22 +620,10 @@ __global__ __launch_bounds__(BAND * KV) void hdiff_fuse2(const S* __restrict__ s
     ld(p1 + 1, 4, nfr);
     ldcf(p1 - 1, ncf);
     C res[V];
-    march_row(o1s, sl1, sr1, fl1, fr1, cf, L01, fx1, res);
+    march_row(rot, o1s, sl1, sr1, fl1, fr1, cf, L01, fx1, res);
     const bool in = col_in && p1 >= 2 && p1 < I + 2;
 #pragma unroll
-    for (int v = 0; v < V; ++v) s1[v] = in ? T::load(T::store(res[v])) : o1s[0][v];
-#pragma unroll
-    for (int v = 0; v < V; ++v) {
-      o1s[0][v] = o1s[1][v];
-      o1s[1][v] = o1s[2][v];
-      sl1[0][v] = sl1[1][v];
-      sl1[1][v] = sl1[2][v];
-      sr1[0][v] = sr1[1][v];
-      sr1[1][v] = sr1[2][v];
-      cfh[2][v] = cfh[1][v];
-      cfh[1][v] = cfh[0][v];
-      cfh[0][v] = cf[v];
-    }
+    for (int v = 0; v < V; ++v) s1[v] = in ? T::load(T::store(res[v])) : o1s[s0][v];
     C* w = ring_at(p1, 0);
 #pragma unroll
     for (int e = 0; e < V; ++e) w[e] = s1[e];
@@ -635,12 +631,18 @@ __global__ __launch_bounds__(BAND * KV) void hdiff_fuse2(const S* __restrict__ s
 
   // ---- warm-up: first-step rows x0 .. x0+3, then the second step's state
   // at row P = x0+2
+  using R0 = std::integral_constant<int, 0>;
+  using R1 = std::integral_constant<int, 1>;
+  using R2 = std::integral_constant<int, 2>;
   C h[4][V];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    step1(x0 + i, h[i]);
-    __syncthreads();
-  }
+  step1(R0{}, x0, h[0]);
+  __syncthreads();
+  step1(R1{}, x0 + 1, h[1]);
+  __syncthreads();
+  step1(R2{}, x0 + 2, h[2]);
+  __syncthreads();
+  step1(R0{}, x0 + 3, h[3]);
+  __syncthreads();
   C o2s[3][V], sl2[3][V], sr2[3][V], fl2[V], fr2[V], L02[V], fx2[V];
   ring_ld(x0 + 1, -1, sl2[0]);
   ring_ld(x0 + 2, -1, sl2[1]);
@@ -653,20 +655,27 @@ __global__ __launch_bounds__(BAND * KV) void hdiff_fuse2(const S* __restrict__ s
     o2s[1][v] = h[3][v];
   }
   const bool store_col = kok && cl >= 2 && cl < BAND - 2 && q < J + 2;
-  // ---- main: first-step row p1, second-step row p2 = p1 - 2
-  for (int p1 = x0 + 4; p1 < x1 + 4; ++p1) {
+  // ---- main: first-step row p1, second-step row p2 = p1 - 2.  Rows go in
+  // threes (the windows' period); the second step's rotation is the row's
+  // place in its three, the first step's one more (four warm-up rows).
+  // The guards are uniform across the workgroup.
+  const int nrows = x1 - x0;
+  auto row = [&](auto rot2, int p1) {
+    using Rot1 = std::integral_constant<int, (decltype(rot2)::value + 1) % 3>;
+    constexpr int s2 = (decltype(rot2)::value + 2) % 3;
     C s1[V];
-    step1(p1, s1);
+    step1(Rot1{}, p1, s1);
     __syncthreads();
 #pragma unroll
-    for (int v = 0; v < V; ++v) o2s[2][v] = s1[v];
-    ring_ld(p1 - 1, -1, sl2[2]);
-    ring_ld(p1 - 1, 1, sr2[2]);
+    for (int v = 0; v < V; ++v) o2s[s2][v] = s1[v];
+    ring_ld(p1 - 1, -1, sl2[s2]);
+    ring_ld(p1 - 1, 1, sr2[s2]);
     ring_ld(p1 - 2, -2, fl2);
     ring_ld(p1 - 2, 2, fr2);
     C res[V];
-    // coefficient of output row p2 - 2 = p1 - 4: the first step's at row p1 - 2
-    march_row(o2s, sl2, sr2, fl2, fr2, cfh[2], L02, fx2, res);
+    // coefficient of output row p2 - 2 = p1 - 4: the first step's at row
+    // p1 - 2, two first-step rotations back
+    march_row(rot2, o2s, sl2, sr2, fl2, fr2, cfh[(Rot1::value + 1) % 3], L02, fx2, res);
     const int p2 = p1 - 2;
     if (store_col) {
       if (next)
@@ -674,15 +683,11 @@ __global__ __launch_bounds__(BAND * KV) void hdiff_fuse2(const S* __restrict__ s
       else
         storev<S, V>(out + (static_cast<int64_t>(p2 - 2) * J + (q - 2)) * K + k0, res);
     }
-#pragma unroll
-    for (int v = 0; v < V; ++v) {
-      o2s[0][v] = o2s[1][v];
-      o2s[1][v] = o2s[2][v];
-      sl2[0][v] = sl2[1][v];
-      sl2[1][v] = sl2[2][v];
-      sr2[0][v] = sr2[1][v];
-      sr2[1][v] = sr2[2][v];
-    }
+  };
+  for (int base = 0; base < nrows; base += 3) {
+    row(R0{}, x0 + 4 + base);
+    if (base + 1 < nrows) row(R1{}, x0 + 5 + base);
+    if (base + 2 < nrows) row(R2{}, x0 + 6 + base);
   }
 }
 

@@ -198,13 +198,14 @@ def test_hdiff_reduced_precision(st, precision, kernel_tol, f64_tol):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16"])
-@pytest.mark.parametrize("rows", [4, 7, 16, 40])
+@pytest.mark.parametrize("rows", [4, 5, 6, 7, 16, 40])
 @pytest.mark.parametrize("I,J,K", [(9, 61, 16), (33, 130, 24), (70, 5, 8)])
 def test_hdiff_two_step_strips(st, hdiff_tblock, precision, rows, I, J, K):
     """The two-step fused pass (hdiff.tblock 2) with strips of 4 to 40
-    output rows (hdiff.fuse_rows), bands of 60 output columns cut by the
-    grid edge and K-vectors past K: bitwise the one-launch-per-step
-    iteration at 4 and 5 steps (a last round of one step)."""
+    output rows (hdiff.fuse_rows; every remainder mod 3, the period of its
+    rotating row windows), bands of 60 output columns cut by the grid edge
+    and K-vectors past K: bitwise the one-launch-per-step iteration at 4 and
+    5 steps (a last round of one step)."""
     from npbench_amd import _native
     dt = {"fp64": torch.float64, "fp32": torch.float32, "bf16": torch.bfloat16}[precision]
     inf, out, coeff = _hdiff_inputs(I, J, K, 5 * I + J + K + rows)
