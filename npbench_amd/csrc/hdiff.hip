@@ -466,55 +466,65 @@ __global__ __launch_bounds__(256) void hdiff_tb(const S* __restrict__ src,
 }
 
 // ---------------------------------------------------------------- two steps
-// hdiff_fuse2<S, VB, BAND, KV>: two iterated steps in one pass, without
-// staging whole blocks.  A workgroup owns BAND in-field columns q = y0 ..
-// y0+BAND-1 (y0 = band (BAND-4)) at KV consecutive k-vectors of VB bytes
-// (lane = (column, k-vector), k fastest, so a wave's loads are 16 columns'
-// contiguous 64-byte k-segments) and marches down a strip of rows.  Each
-// lane runs hdiff_march's one-column march twice over: the first step at
-// in-field row p1 from the input, the second at row p1 - 2 from first-step
-// values -- its own column's from its registers, the two columns either
-// side of it from neighbouring lanes through a four-row LDS ring (written
-// before and read after one barrier per row).  First-step values outside
-// the field's interior are the input's (the halo the iterated form holds
-// fixed), interior ones are rounded through the storage type as the
-// iterated form's stores round them, and every Laplacian and flux has the
-// reference's operands in the reference's order, so the output is bitwise
-// two launches of hdiff_march.  The band's outer two columns on each side
-// give first-step values only (BAND - 4 columns of output), and a strip of
-// S output rows computes S + 4 first-step rows.
-template <class S, int VB, int BAND, int KV>
-__global__ __launch_bounds__(BAND * KV) void hdiff_fuse2(const S* __restrict__ src,
-                                                         const S* __restrict__ coeff,
-                                                         S* __restrict__ out, S* __restrict__ next,
-                                                         int I, int J, int K, int bands,
-                                                         int kgroups, int strip) {
+// hdiff_fuse2w<S, VB, W, BAND, KV>: two iterated steps in one pass, without
+// staging whole blocks.  A workgroup owns BAND in-field columns y0 ..
+// y0+BAND-1 (y0 = band (BAND-4)) at KV consecutive k-vectors of VB bytes;
+// a lane owns W adjacent columns at one k-vector (lane = (column group,
+// k-vector), k fastest, so a wave load covers KV x VB contiguous bytes of
+// each of its columns) and marches down a strip of rows, running
+// hdiff_march's W-column row twice over: the first step at in-field row p1
+// from the input, the second at row p1 - 2 from first-step values -- its
+// own columns' from its registers, the side and far columns either side of
+// the group from neighbouring lanes through a four-row LDS ring (written
+// before and read after one barrier per row).  A W-column row shares the
+// Laplacian and the j-flux between the lane's own columns.  First-step
+// values outside the field's interior are the input's (the halo the
+// iterated form holds fixed), interior ones are rounded through the storage
+// type as the iterated form's stores round them, and every Laplacian and
+// flux has the reference's operands in the reference's order, so the output
+// is bitwise two launches of hdiff_march.  The band's outer two columns on
+// each side give first-step values only (BAND - 4 columns of output), a
+// strip of S output rows computes S + 4 first-step rows, and the three-row
+// windows rotate through fixed register slots (rows in threes) instead of
+// being copied.
+template <class S, int VB, int W, int BAND, int KV>
+__global__ __launch_bounds__(BAND / W * KV) void hdiff_fuse2w(const S* __restrict__ src,
+                                                              const S* __restrict__ coeff,
+                                                              S* __restrict__ out,
+                                                              S* __restrict__ next, int I, int J,
+                                                              int K, int bands, int kgroups,
+                                                              int strip) {
   using T = Storage<S>;
   using C = typename T::C;
   constexpr int V = VB / static_cast<int>(sizeof(S));
+  static_assert(BAND % W == 0, "whole column groups per band");
   using Raw = RawVec<S, V>;
   __shared__ C ring[4][BAND * KV * V];  // first-step rows: [row % 4][column][k-vector][v]
   const int b = xcd_tile(blockIdx.x, gridDim.x);
   const int band = b % bands, rest = b / bands;
   const int kg = rest % kgroups, sidx = rest / kgroups;
-  const int cl = threadIdx.x / KV, kv = threadIdx.x % KV;
+  const int cg = threadIdx.x / KV, kv = threadIdx.x % KV;
+  const int c0 = cg * W;                  // the group's first column in the band
   const int y0 = band * (BAND - 4);
-  const int q = y0 + cl;  // this lane's in-field column
+  const int qa = y0 + c0;                 // in-field column of own column 0
   const int kraw = (kg * KV + kv) * V;
   const bool kok = kraw + V <= K;
   const int k0 = kok ? kraw : K - V;  // lanes past K compute a copy and store nothing
   const int x0 = sidx * strip, x1 = min(x0 + strip, I);  // second-step output rows
   const int64_t si = static_cast<int64_t>(J + 4) * K;
   auto colo = [&](int qq) { return static_cast<int64_t>(min(max(qq, 0), J + 3)) * K + k0; };
-  // in-field columns q-2 .. q+2, clamped (clamped columns feed only values
-  // that are replaced or not stored)
-  const int64_t cq[5] = {colo(q - 2), colo(q - 1), colo(q), colo(q + 1), colo(q + 2)};
+  // in-field columns qa-2 .. qa+W+1 (far left, side left, own, side right,
+  // far right), clamped (clamped columns feed only values that are replaced
+  // or not stored)
+  int64_t cq[W + 4];
+#pragma unroll
+  for (int c = 0; c < W + 4; ++c) cq[c] = colo(qa - 2 + c);
   auto ld = [&](int p, int c, Raw& r) {
     loadraw<S, V>(src + static_cast<int64_t>(min(max(p, 0), I + 3)) * si + cq[c], r);
   };
-  const S* cfbase = coeff + static_cast<int64_t>(min(max(q - 2, 0), J - 1)) * K + k0;
-  auto ldcf = [&](int x, Raw& r) {
-    loadraw<S, V>(cfbase + static_cast<int64_t>(min(max(x, 0), I - 1)) * J * K, r);
+  auto ldcf = [&](int x, int w, Raw& r) {
+    loadraw<S, V>(coeff + (static_cast<int64_t>(min(max(x, 0), I - 1)) * J +
+                           min(max(qa + w - 2, 0), J - 1)) * K + k0, r);
   };
   auto lap = [](C c, C ip, C im, C jp, C jm) {
     C nb = ip + im;
@@ -522,111 +532,154 @@ __global__ __launch_bounds__(BAND * KV) void hdiff_fuse2(const S* __restrict__ s
     nb = nb + jm;
     return static_cast<C>(4.0) * c - nb;
   };
-  const bool col_in = q >= 2 && q < J + 2;
-  // one row of the one-column march: own[s0], own[s1], own[s2] = rows p,
-  // p+1, p+2 of the column, sl/sr[s0 .. s2] = rows p-1, p, p+1 of columns
-  // q-1 / q+1 (s0 = the rotation rot, s1 = rot+1, s2 = rot+2, mod 3: the
-  // three-row windows rotate through fixed slots instead of being copied),
-  // fl/fr = row p of q-2 / q+2; L0, fx carried
-  auto march_row = [&](auto rot, const C (&own)[3][V], const C (&sl)[3][V],
-                       const C (&sr)[3][V], const C (&fl)[V], const C (&fr)[V],
-                       const C (&cf)[V], C (&L0)[V], C (&fx)[V], C (&res)[V]) {
-    constexpr int s0 = decltype(rot)::value % 3, s1 = (s0 + 1) % 3, s2 = (s0 + 2) % 3;
+  bool col_in[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) col_in[w] = qa + w >= 2 && qa + w < J + 2;
+  using Own = C[3][W][V];   // own columns' rows p, p+1, p+2 at slots rot, rot+1, rot+2 (mod 3)
+  using Side = C[3][2][V];  // side columns' (0 left, 1 right) rows p-1, p, p+1, same slots
+  // one W-column row at rotation rot (hdiff_march's row): far[f] = row p of
+  // the far columns; L0, fx carried
+  auto mrow = [&](auto rot, const Own& own, const Side& sd, const C (&far)[2][V],
+                  const C (&cf)[W][V], C (&L0)[W][V], C (&fx)[W][V], C (&res)[W][V]) {
+    constexpr int a0 = decltype(rot)::value % 3, a1 = (a0 + 1) % 3, a2 = (a0 + 2) % 3;
 #pragma unroll
     for (int v = 0; v < V; ++v) {
-      const C c = own[s0][v];
-      const C lcl = lap(sl[s1][v], sl[s2][v], sl[s0][v], c, fl[v]);
-      const C lcr = lap(sr[s1][v], sr[s2][v], sr[s0][v], fr[v], c);
-      const C fy0 = limit<C>(L0[v] - lcl, c - sl[s1][v]);
-      const C fy1 = limit<C>(lcr - L0[v], sr[s1][v] - c);
-      const C Lip = lap(own[s1][v], own[s2][v], c, sr[s2][v], sl[s2][v]);
-      const C flx_hi = limit<C>(Lip - L0[v], own[s1][v] - c);
-      C dv = flx_hi - fx[v];
-      dv = dv + fy1;
-      dv = dv - fy0;
-      res[v] = c - cf[v] * dv;
-      L0[v] = Lip;
-      fx[v] = flx_hi;
-    }
-  };
-  // march start at row P: L0 = lap(P), fx = flux across (P-1 | P)
-  auto march_init = [&](const C (&om2)[V], const C (&om1)[V], const C (&o0)[V],
-                        const C (&o1)[V], const C (&slm)[V], const C (&sl0)[V],
-                        const C (&srm)[V], const C (&sr0)[V], C (&L0)[V], C (&fx)[V]) {
+      C lc[W + 2], cv[W + 2];
+      lc[0] = lap(sd[a1][0][v], sd[a2][0][v], sd[a0][0][v], own[a0][0][v], far[0][v]);
+      lc[W + 1] = lap(sd[a1][1][v], sd[a2][1][v], sd[a0][1][v], far[1][v], own[a0][W - 1][v]);
+      cv[0] = sd[a1][0][v];
+      cv[W + 1] = sd[a1][1][v];
 #pragma unroll
-    for (int v = 0; v < V; ++v) {
-      L0[v] = lap(o0[v], o1[v], om1[v], sr0[v], sl0[v]);
-      const C Lim = lap(om1[v], o0[v], om2[v], srm[v], slm[v]);
-      fx[v] = limit<C>(L0[v] - Lim, o0[v] - om1[v]);
+      for (int w = 0; w < W; ++w) {
+        lc[w + 1] = L0[w][v];
+        cv[w + 1] = own[a0][w][v];
+      }
+      C fy[W + 1];
+#pragma unroll
+      for (int c = 0; c <= W; ++c) fy[c] = limit<C>(lc[c + 1] - lc[c], cv[c + 1] - cv[c]);
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const C c = own[a0][w][v];
+        const C rp1 = w + 1 < W ? own[a1][w + 1][v] : sd[a2][1][v];  // (p+1, q+1)
+        const C lp1 = w > 0 ? own[a1][w - 1][v] : sd[a2][0][v];      // (p+1, q-1)
+        const C Lip = lap(own[a1][w][v], own[a2][w][v], c, rp1, lp1);
+        const C flx_hi = limit<C>(Lip - L0[w][v], own[a1][w][v] - c);
+        C dv = flx_hi - fx[w][v];
+        dv = dv + fy[w + 1];
+        dv = dv - fy[w];
+        res[w][v] = c - cf[w][v] * dv;
+        L0[w][v] = Lip;
+        fx[w][v] = flx_hi;
+      }
     }
   };
-  auto ring_at = [&](int row, int dc) {  // row's slot, column cl + dc (clamped into the band)
-    const int c = min(max(cl + dc, 0), BAND - 1);
-    return &ring[row & 3][(c * KV + kv) * V];
+  // march start at row P (hdiff_march's): own rows P-2 .. P+1 in om2, om1,
+  // own[0], own[1] (slots 0, 1); side rows P-1, P in sd[0], sd[1]
+  auto minit = [&](const C (&om2)[W][V], const C (&om1)[W][V], const Own& own, const Side& sd,
+                   C (&L0)[W][V], C (&fx)[W][V]) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const C c = own[0][w][v];
+        const C rp = w + 1 < W ? own[0][w + 1][v] : sd[1][1][v];  // (P, q+1)
+        const C lp = w > 0 ? own[0][w - 1][v] : sd[1][0][v];      // (P, q-1)
+        const C rm = w + 1 < W ? om1[w + 1][v] : sd[0][1][v];     // (P-1, q+1)
+        const C lm = w > 0 ? om1[w - 1][v] : sd[0][0][v];         // (P-1, q-1)
+        L0[w][v] = lap(c, own[1][w][v], om1[w][v], rp, lp);
+        const C Lim = lap(om1[w][v], c, om2[w][v], rm, lm);
+        fx[w][v] = limit<C>(L0[w][v] - Lim, c - om1[w][v]);
+      }
+    }
   };
-  auto ring_ld = [&](int row, int dc, C (&v)[V]) {
-    const C* p = ring_at(row, dc);
+  auto ring_at = [&](int row, int c) {  // row's slot, band column c (clamped into the band)
+    const int cc = min(max(c, 0), BAND - 1);
+    return &ring[row & 3][(cc * KV + kv) * V];
+  };
+  auto ring_ld = [&](int row, int c, C (&v)[V]) {
+    const C* p = ring_at(row, c);
 #pragma unroll
     for (int e = 0; e < V; ++e) v[e] = p[e];
   };
 
   // ---- first step: state at row x0
-  C o1s[3][V], sl1[3][V], sr1[3][V], fl1[V], fr1[V], L01[V], fx1[V];
+  Own o1;
+  Side s1;
+  C far1[2][V], L01[W][V], fx1[W][V];
   {
-    Raw r[4], rl[2], rr[2];
+    Raw r[W][4], rs[2][2];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) ld(x0 - 2 + i, 2, r[i]);
+    for (int w = 0; w < W; ++w)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) ld(x0 - 2 + i, w + 2, r[w][i]);
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-      ld(x0 - 1 + i, 1, rl[i]);
-      ld(x0 - 1 + i, 3, rr[i]);
+      ld(x0 - 1 + i, 1, rs[0][i]);
+      ld(x0 - 1 + i, W + 2, rs[1][i]);
     }
-    C om2[V], om1[V];
-    unpackv<S, V>(r[0], om2);
-    unpackv<S, V>(r[1], om1);
-    unpackv<S, V>(r[2], o1s[0]);
-    unpackv<S, V>(r[3], o1s[1]);
-    unpackv<S, V>(rl[0], sl1[0]);
-    unpackv<S, V>(rl[1], sl1[1]);
-    unpackv<S, V>(rr[0], sr1[0]);
-    unpackv<S, V>(rr[1], sr1[1]);
-    march_init(om2, om1, o1s[0], o1s[1], sl1[0], sl1[1], sr1[0], sr1[1], L01, fx1);
+    C om2[W][V], om1[W][V];
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      unpackv<S, V>(r[w][0], om2[w]);
+      unpackv<S, V>(r[w][1], om1[w]);
+      unpackv<S, V>(r[w][2], o1[0][w]);
+      unpackv<S, V>(r[w][3], o1[1][w]);
+    }
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+      unpackv<S, V>(rs[f][0], s1[0][f]);
+      unpackv<S, V>(rs[f][1], s1[1][f]);
+    }
+    minit(om2, om1, o1, s1, L01, fx1);
   }
-  Raw nown, nsl, nsr, nfl, nfr, ncf;
-  ld(x0 + 2, 2, nown);
-  ld(x0 + 1, 1, nsl);
-  ld(x0 + 1, 3, nsr);
-  ld(x0, 0, nfl);
-  ld(x0, 4, nfr);
-  ldcf(x0 - 2, ncf);
+  Raw nown[W], nsd[2], nfar[2], ncf[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) ld(x0 + 2, w + 2, nown[w]);
+  ld(x0 + 1, 1, nsd[0]);
+  ld(x0 + 1, W + 2, nsd[1]);
+  ld(x0, 0, nfar[0]);
+  ld(x0, W + 3, nfar[1]);
+#pragma unroll
+  for (int w = 0; w < W; ++w) ldcf(x0 - 2, w, ncf[w]);
   // coefficients of the last three first-step rows: row p1's in slot
-  // rot % 3 of its step (rot as in march_row)
-  C cfh[3][V];
-  // one first-step row p1 at rotation rot: its value at column q, into the
-  // ring
-  auto step1 = [&](auto rot, int p1, C (&s1)[V]) {
-    constexpr int s0 = decltype(rot)::value % 3, s2 = (s0 + 2) % 3;
-    C (&cf)[V] = cfh[s0];
-    unpackv<S, V>(nown, o1s[s2]);
-    unpackv<S, V>(nsl, sl1[s2]);
-    unpackv<S, V>(nsr, sr1[s2]);
-    unpackv<S, V>(nfl, fl1);
-    unpackv<S, V>(nfr, fr1);
-    unpackv<S, V>(ncf, cf);
-    ld(p1 + 3, 2, nown);  // next row's loads (rows clamped into the field)
-    ld(p1 + 2, 1, nsl);
-    ld(p1 + 2, 3, nsr);
-    ld(p1 + 1, 0, nfl);
-    ld(p1 + 1, 4, nfr);
-    ldcf(p1 - 1, ncf);
-    C res[V];
-    march_row(rot, o1s, sl1, sr1, fl1, fr1, cf, L01, fx1, res);
-    const bool in = col_in && p1 >= 2 && p1 < I + 2;
+  // rot % 3 of its step
+  C cfh[3][W][V];
+  // one first-step row p1 at rotation rot: its values at the own columns,
+  // into the ring
+  auto step1 = [&](auto rot, int p1, C (&sv)[W][V]) {
+    constexpr int a0 = decltype(rot)::value % 3, a2 = (a0 + 2) % 3;
+    C (&cf)[W][V] = cfh[a0];
 #pragma unroll
-    for (int v = 0; v < V; ++v) s1[v] = in ? T::load(T::store(res[v])) : o1s[s0][v];
-    C* w = ring_at(p1, 0);
+    for (int w = 0; w < W; ++w) {
+      unpackv<S, V>(nown[w], o1[a2][w]);
+      unpackv<S, V>(ncf[w], cf[w]);
+    }
 #pragma unroll
-    for (int e = 0; e < V; ++e) w[e] = s1[e];
+    for (int f = 0; f < 2; ++f) {
+      unpackv<S, V>(nsd[f], s1[a2][f]);
+      unpackv<S, V>(nfar[f], far1[f]);
+    }
+    // next row's loads (rows clamped into the field)
+#pragma unroll
+    for (int w = 0; w < W; ++w) ld(p1 + 3, w + 2, nown[w]);
+    ld(p1 + 2, 1, nsd[0]);
+    ld(p1 + 2, W + 2, nsd[1]);
+    ld(p1 + 1, 0, nfar[0]);
+    ld(p1 + 1, W + 3, nfar[1]);
+#pragma unroll
+    for (int w = 0; w < W; ++w) ldcf(p1 - 1, w, ncf[w]);
+    C res[W][V];
+    mrow(rot, o1, s1, far1, cf, L01, fx1, res);
+    const bool rin = p1 >= 2 && p1 < I + 2;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const bool in = rin && col_in[w];
+#pragma unroll
+      for (int v = 0; v < V; ++v) sv[w][v] = in ? T::load(T::store(res[w][v])) : o1[a0][w][v];
+      C* wp = ring_at(p1, c0 + w);
+#pragma unroll
+      for (int e = 0; e < V; ++e) wp[e] = sv[w][e];
+    }
   };
 
   // ---- warm-up: first-step rows x0 .. x0+3, then the second step's state
@@ -634,7 +687,7 @@ __global__ __launch_bounds__(BAND * KV) void hdiff_fuse2(const S* __restrict__ s
   using R0 = std::integral_constant<int, 0>;
   using R1 = std::integral_constant<int, 1>;
   using R2 = std::integral_constant<int, 2>;
-  C h[4][V];
+  C h[4][W][V];
   step1(R0{}, x0, h[0]);
   __syncthreads();
   step1(R1{}, x0 + 1, h[1]);
@@ -643,45 +696,56 @@ __global__ __launch_bounds__(BAND * KV) void hdiff_fuse2(const S* __restrict__ s
   __syncthreads();
   step1(R0{}, x0 + 3, h[3]);
   __syncthreads();
-  C o2s[3][V], sl2[3][V], sr2[3][V], fl2[V], fr2[V], L02[V], fx2[V];
-  ring_ld(x0 + 1, -1, sl2[0]);
-  ring_ld(x0 + 2, -1, sl2[1]);
-  ring_ld(x0 + 1, 1, sr2[0]);
-  ring_ld(x0 + 2, 1, sr2[1]);
-  march_init(h[0], h[1], h[2], h[3], sl2[0], sl2[1], sr2[0], sr2[1], L02, fx2);
+  Own o2;
+  Side s2;
+  C far2[2][V], L02[W][V], fx2[W][V];
+  ring_ld(x0 + 1, c0 - 1, s2[0][0]);
+  ring_ld(x0 + 2, c0 - 1, s2[1][0]);
+  ring_ld(x0 + 1, c0 + W, s2[0][1]);
+  ring_ld(x0 + 2, c0 + W, s2[1][1]);
 #pragma unroll
-  for (int v = 0; v < V; ++v) {
-    o2s[0][v] = h[2][v];
-    o2s[1][v] = h[3][v];
-  }
-  const bool store_col = kok && cl >= 2 && cl < BAND - 2 && q < J + 2;
-  // ---- main: first-step row p1, second-step row p2 = p1 - 2.  Rows go in
-  // threes (the windows' period); the second step's rotation is the row's
-  // place in its three, the first step's one more (four warm-up rows).
-  // The guards are uniform across the workgroup.
+  for (int w = 0; w < W; ++w)
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      o2[0][w][v] = h[2][w][v];
+      o2[1][w][v] = h[3][w][v];
+    }
+  minit(h[0], h[1], o2, s2, L02, fx2);
+  bool store_col[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w)
+    store_col[w] = kok && c0 + w >= 2 && c0 + w < BAND - 2 && qa + w < J + 2;
+  // ---- main: first-step row p1, second-step row p2 = p1 - 2, in threes
+  // (the windows' period; the first step's rotation is one more than the
+  // second's after four warm-up rows).  The guards are uniform across the
+  // workgroup.
   const int nrows = x1 - x0;
   auto row = [&](auto rot2, int p1) {
     using Rot1 = std::integral_constant<int, (decltype(rot2)::value + 1) % 3>;
-    constexpr int s2 = (decltype(rot2)::value + 2) % 3;
-    C s1[V];
-    step1(Rot1{}, p1, s1);
+    constexpr int a2 = (decltype(rot2)::value + 2) % 3;
+    C sv[W][V];
+    step1(Rot1{}, p1, sv);
     __syncthreads();
 #pragma unroll
-    for (int v = 0; v < V; ++v) o2s[s2][v] = s1[v];
-    ring_ld(p1 - 1, -1, sl2[s2]);
-    ring_ld(p1 - 1, 1, sr2[s2]);
-    ring_ld(p1 - 2, -2, fl2);
-    ring_ld(p1 - 2, 2, fr2);
-    C res[V];
-    // coefficient of output row p2 - 2 = p1 - 4: the first step's at row
+    for (int w = 0; w < W; ++w)
+#pragma unroll
+      for (int v = 0; v < V; ++v) o2[a2][w][v] = sv[w][v];
+    ring_ld(p1 - 1, c0 - 1, s2[a2][0]);
+    ring_ld(p1 - 1, c0 + W, s2[a2][1]);
+    ring_ld(p1 - 2, c0 - 2, far2[0]);
+    ring_ld(p1 - 2, c0 + W + 1, far2[1]);
+    C res[W][V];
+    // coefficients of output row p2 - 2 = p1 - 4: the first step's at row
     // p1 - 2, two first-step rotations back
-    march_row(rot2, o2s, sl2, sr2, fl2, fr2, cfh[(Rot1::value + 1) % 3], L02, fx2, res);
+    mrow(rot2, o2, s2, far2, cfh[(Rot1::value + 1) % 3], L02, fx2, res);
     const int p2 = p1 - 2;
-    if (store_col) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      if (!store_col[w]) continue;
       if (next)
-        storev<S, V>(next + static_cast<int64_t>(p2) * si + cq[2], res);
+        storev<S, V>(next + static_cast<int64_t>(p2) * si + cq[w + 2], res[w]);
       else
-        storev<S, V>(out + (static_cast<int64_t>(p2 - 2) * J + (q - 2)) * K + k0, res);
+        storev<S, V>(out + (static_cast<int64_t>(p2 - 2) * J + (qa + w - 2)) * K + k0, res[w]);
     }
   };
   for (int base = 0; base < nrows; base += 3) {
@@ -764,9 +828,12 @@ int run_hdiff(const S* in, S* out, const S* coeff, int I, int J, int K, int64_t 
                        K);
     NPBX_LAUNCHED("hdiff_copy_halo");
   }
-  // Two steps in one pass (hdiff_fuse2): 64 columns (60 of output) x 4
-  // k-vectors per workgroup, strips of hdiff.fuse_rows output rows.
-  constexpr int kFuseBand = 64, kFuseKV = 4;
+  // Two steps in one pass (hdiff_fuse2w): 64 columns (60 of output) per
+  // workgroup, strips of hdiff.fuse_rows output rows.  BF16: two columns
+  // per lane (8 k-vectors of 8 bytes); FP32 and FP64: one (4 k-vectors of
+  // 16 bytes: two columns of 16-byte vectors run one wave per SIMD).
+  constexpr int kFuseBand = 64;
+  constexpr int kFuseW = sizeof(S) == 2 ? 2 : 1, kFuseKV = sizeof(S) == 2 ? 8 : 4;
   constexpr int kFuseVB = sizeof(S) == 2 ? 8 : 16;
   constexpr int kFuseV = kFuseVB / static_cast<int>(sizeof(S));
   auto fuse2 = [&](const S* from, S* to) -> int {
@@ -775,10 +842,10 @@ int run_hdiff(const S* in, S* out, const S* coeff, int I, int J, int K, int64_t 
     const int strip = hdiff_fuse_rows();
     const int64_t nb = int64_t(bands) * kgroups * cdiv(I, strip);
     NPBX_REQUIRE(nb < (int64_t(1) << 31), "hdiff: grid too large");
-    hipLaunchKernelGGL((hdiff_fuse2<S, kFuseVB, kFuseBand, kFuseKV>),
-                       dim3(static_cast<unsigned>(nb)), dim3(kFuseBand * kFuseKV), 0, stream, from,
-                       coeff, out, to, I, J, K, bands, kgroups, strip);
-    NPBX_LAUNCHED("hdiff_fuse2");
+    hipLaunchKernelGGL((hdiff_fuse2w<S, kFuseVB, kFuseW, kFuseBand, kFuseKV>),
+                       dim3(static_cast<unsigned>(nb)), dim3(kFuseBand / kFuseW * kFuseKV), 0,
+                       stream, from, coeff, out, to, I, J, K, bands, kgroups, strip);
+    NPBX_LAUNCHED("hdiff_fuse2w");
     return NPBX_OK;
   };
   const S* src = in;
