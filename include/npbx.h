@@ -221,10 +221,11 @@ int npbx_pack_cols(int dtype, const void *A, const void *B, int64_t n,
  * sweep's planes per workgroup / planes loaded ahead (0: 16/4, 1: 16/8,
  * 2: 32/8, 3: 8/4), "hdiff.tblock" = iterated hdiff steps fused per HBM round
  * trip (1, the default, 2: the two-step register march with strips of
- * "hdiff.fuse_rows" output rows (default 16), or 4: LDS blocks; BF16 at
- * most 2), "adi.path" = ADI launches (0,
- * the default: by size; 1: all time steps in one single-workgroup launch, up
- * to 512 lines; 2: one launch per half step), "vadv.path" = vadv kernel (0,
+ * "hdiff.fuse_rows" output rows (0, the default: 16 for BF16, 32
+ * otherwise), or 4: LDS blocks; BF16 at most 2), "adi.path" = ADI launches
+ * (0, the default: by size; 1: all time steps in one single-workgroup
+ * launch, up to 512 lines; 2: one launch per half step), "vadv.path" = vadv
+ * kernel (0,
  * the default: by shape; 1: c' and d' through a device workspace; 2: c' and
  * d' in LDS where K <= 160 and columns hold whole 16-byte vectors; 3: c' in
  * registers, d' in LDS, 80 < K <= 160; 4: loads coalesced and staged in

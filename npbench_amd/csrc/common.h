@@ -150,7 +150,7 @@ void set_hdiff_tblock(int tb);
 int hdiff_shape();
 void set_hdiff_shape(int shape);
 // Output rows per workgroup strip of the two-step fused hdiff
-// ("hdiff.fuse_rows", 4 .. 256).
+// ("hdiff.fuse_rows", 4 .. 256; 0 = by storage type).
 int hdiff_fuse_rows();
 void set_hdiff_fuse_rows(int rows);
 int hdiff_shape_count();

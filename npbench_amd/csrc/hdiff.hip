@@ -828,18 +828,20 @@ int run_hdiff(const S* in, S* out, const S* coeff, int I, int J, int K, int64_t 
                        K);
     NPBX_LAUNCHED("hdiff_copy_halo");
   }
-  // Two steps in one pass (hdiff_fuse2w): 64 columns (60 of output) per
-  // workgroup, strips of hdiff.fuse_rows output rows.  BF16: two columns
-  // per lane (8 k-vectors of 8 bytes); FP32 and FP64: one (4 k-vectors of
-  // 16 bytes: two columns of 16-byte vectors run one wave per SIMD).
-  constexpr int kFuseBand = 64;
-  constexpr int kFuseW = sizeof(S) == 2 ? 2 : 1, kFuseKV = sizeof(S) == 2 ? 8 : 4;
+  // Two steps in one pass (hdiff_fuse2w), strips of hdiff.fuse_rows output
+  // rows (0: 16 for BF16, 32 otherwise).  BF16: 64 columns (60 of output)
+  // per workgroup, two per lane, 8 k-vectors of 8 bytes; FP32 and FP64: 32
+  // columns (28 of output), one per lane (two columns of 16-byte vectors
+  // run one wave per SIMD), 8 k-vectors of 16 bytes.  The shapes measured
+  // fastest at 256 x 256 x 160.
+  constexpr int kFuseBand = sizeof(S) == 2 ? 64 : 32;
+  constexpr int kFuseW = sizeof(S) == 2 ? 2 : 1, kFuseKV = 8;
   constexpr int kFuseVB = sizeof(S) == 2 ? 8 : 16;
   constexpr int kFuseV = kFuseVB / static_cast<int>(sizeof(S));
   auto fuse2 = [&](const S* from, S* to) -> int {
     const int bands = static_cast<int>(cdiv(J, kFuseBand - 4));
     const int kgroups = static_cast<int>(cdiv(K / kFuseV, kFuseKV));
-    const int strip = hdiff_fuse_rows();
+    const int strip = hdiff_fuse_rows() > 0 ? hdiff_fuse_rows() : (sizeof(S) == 2 ? 16 : 32);
     const int64_t nb = int64_t(bands) * kgroups * cdiv(I, strip);
     NPBX_REQUIRE(nb < (int64_t(1) << 31), "hdiff: grid too large");
     hipLaunchKernelGGL((hdiff_fuse2w<S, kFuseVB, kFuseW, kFuseBand, kFuseKV>),
@@ -890,7 +892,7 @@ int run_hdiff(const S* in, S* out, const S* coeff, int I, int J, int K, int64_t 
 
 int hdiff_tblock() { return g_hdiff_tblock.load(); }
 void set_hdiff_tblock(int tb) { g_hdiff_tblock.store(tb); }
-std::atomic<int> g_hdiff_fuse_rows{16};
+std::atomic<int> g_hdiff_fuse_rows{0};
 int hdiff_fuse_rows() { return g_hdiff_fuse_rows.load(); }
 void set_hdiff_fuse_rows(int rows) { g_hdiff_fuse_rows.store(rows); }
 int hdiff_shape() { return g_hdiff_shape.load(); }

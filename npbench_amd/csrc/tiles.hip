@@ -1447,7 +1447,8 @@ extern "C" int npbx_set_tuning(const char* key, int64_t value) {
     return NPBX_OK;
   }
   if (std::strcmp(key, "hdiff.fuse_rows") == 0) {
-    NPBX_REQUIRE(value >= 4 && value <= 256, "set_tuning: hdiff.fuse_rows must be in [4, 256]");
+    NPBX_REQUIRE(value == 0 || (value >= 4 && value <= 256),
+                 "set_tuning: hdiff.fuse_rows must be 0 or in [4, 256]");
     set_hdiff_fuse_rows(static_cast<int>(value));
     return NPBX_OK;
   }
